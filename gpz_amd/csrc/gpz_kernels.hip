@@ -1,0 +1,1174 @@
+// gpz_kernels.hip -- MI355X (gfx950) kernels of the GPZ particle compressor
+// and the C ABI declared in include/gpz.h.
+//
+// One wave64 owns one 1024-point group.  Lane l holds the 16 points whose
+// logical position in CUB's blocked sort input is L = 16*l + k, i.e. slot
+// j = 32*(16*(l&1) + k) + (l>>1) of the group (the reference loads slot
+// 32*i + t into thread t item i, src/cudaWarpCompress.cu:104-118, and
+// BlockRadixSort sees thread t item i at position 32*t + i, :213).  Loading
+// in that order lets a stable LSD radix sort reproduce the reference's tie
+// order without a transpose; each load instruction still covers two full
+// 128-byte lines.
+//
+// Pipeline per call (all on one stream, no host round trip):
+//   seg_sample / seg_decide : device-side computeSegSize
+//                             (src/cudaWarpCompress.cu:1424-1557)
+//   encode                  : quantise, block id, LDS radix sort, runs,
+//                             bit planes staged in LDS, copied to a fixed
+//                             9344-byte slot (src/cudaWarpCompress.cu:58-435)
+//   scan                    : exclusive scan of payload sizes, header
+//                             (src/cudaWarpCompress.cu:1658-1683)
+//   compact                 : meta + payload into the final stream
+//                             (src/cudaWarpCompress.cu:32-56,1686-1688)
+//   decode                  : inverse, one wave per group
+//                             (src/cudaWarpCompress.cu:798-1214)
+//
+// Arithmetic follows the reference bit for bit: explicit fma, IEEE division
+// and sqrt, saturating float->unsigned conversion (CUDA cvt.rzi), shifts by
+// >= the width give 0 (PTX shr), u32 (FP32) / u64 (FP64) wrap-around.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "gpz.h"
+
+namespace {
+
+constexpr uint32_t kGroup = 1024;
+constexpr uint32_t kSlot = 9344;       // max payload bytes of one group
+constexpr uint32_t kSortWords = 1280;  // 1024 words + 4 pad words per 16
+constexpr uint32_t kSampleLen = 256;   // src/cudaWarpCompress.cu:1425-1426
+constexpr uint32_t kSampleBlocks = 1024;
+constexpr uint32_t kMinMaxBlocks = 1024;
+
+template <typename T> struct Prec;
+template <> struct Prec<float> {
+  using Q = uint32_t;
+  static constexpr uint32_t kHeader = 24, kMeta = 36, kPad = 4;
+  __device__ static float lo() { return 3.402823466e+38f; }
+};
+template <> struct Prec<double> {
+  using Q = uint64_t;
+  static constexpr uint32_t kHeader = 32, kMeta = 64, kPad = 8;
+  __device__ static double lo() { return 1.7976931348623157e+308; }
+};
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ uint32_t sat_u32(float f) {
+  if (!(f >= 1.0f)) return 0u;
+  if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+  return static_cast<uint32_t>(f);
+}
+__device__ __forceinline__ uint32_t sat_u32(double f) {
+  if (!(f >= 1.0)) return 0u;
+  if (f >= 4294967296.0) return 0xFFFFFFFFu;
+  return static_cast<uint32_t>(f);
+}
+__device__ __forceinline__ uint64_t sat_u64(double f) {
+  if (!(f >= 1.0)) return 0ull;
+  if (f >= 18446744073709551616.0) return ~0ull;
+  return static_cast<uint64_t>(f);
+}
+__device__ __forceinline__ uint32_t eff_bits(uint32_t x) {
+  return x ? 32u - __builtin_clz(x) : 0u;
+}
+__device__ __forceinline__ uint32_t eff_bits(uint64_t x) {
+  return x ? 64u - __builtin_clzll(x) : 0u;
+}
+__device__ __forceinline__ uint32_t shr(uint32_t v, uint32_t s) {
+  return s >= 32 ? 0u : (v >> s);
+}
+__device__ __forceinline__ uint64_t shr(uint64_t v, uint32_t s) {
+  return s >= 64 ? 0ull : (v >> s);
+}
+template <typename Q> __device__ __forceinline__ Q udiv(Q a, Q b) {
+  return b ? a / b : ~Q(0);
+}
+template <typename Q> __device__ __forceinline__ Q urem(Q a, Q b) {
+  return b ? a % b : a;
+}
+
+__device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double dmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ double dmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float dfma(float a, float b, float c) {
+  return fmaf(a, b, c);
+}
+__device__ __forceinline__ double dfma(double a, double b, double c) {
+  return fma(a, b, c);
+}
+__device__ __forceinline__ float dceil(float a) { return ceilf(a); }
+__device__ __forceinline__ double dceil(double a) { return ceil(a); }
+
+// wave64 collectives (ds_bpermute based shuffles)
+template <typename V> __device__ __forceinline__ V wave_min(V v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v = dmin(v, __shfl_xor(v, m));
+  return v;
+}
+template <typename V> __device__ __forceinline__ V wave_max(V v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v = dmax(v, __shfl_xor(v, m));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+    const uint32_t o = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m));
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), d));
+    if (lane >= static_cast<uint32_t>(d)) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, uint32_t lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t t = static_cast<uint64_t>(
+        __shfl_up(static_cast<long long>(v), d));
+    if (lane >= static_cast<uint32_t>(d)) v += t;
+  }
+  return v;
+}
+
+// Section sizes (src/cudaWarpCompress.cu:324-329; FP64 3D
+// src/cudaWarpCompressDouble.cu:325-329, FP64 2D :686-692).
+template <typename T, int DIM>
+__device__ __host__ __forceinline__ void section_sizes(
+    uint32_t U, uint32_t db, uint32_t cb, uint32_t rb, uint32_t* D,
+    uint32_t* C, uint32_t* R) {
+  const uint32_t B = (U + 7u) >> 3;
+  if (sizeof(T) == 4) {
+    *D = (B * db + 3u) & ~3u;
+    *C = (B * cb + 3u) & ~3u;
+    *R = 128u * rb;
+  } else {
+    *D = (B * db + 7u) & ~7u;
+    *C = DIM == 3 ? ((B * cb + 7u) & ~7u) : ((B * cb + 3u) & ~3u);
+    *R = 128u * rb;
+  }
+}
+
+// Block counts per axis from a group's extrema
+// (src/cudaWarpCompress.cu:150-154; FP64 3D: u32 cast, 2D encoder: u64 cast,
+// src/cudaWarpCompressDouble.cu:151-154,525-526).
+template <typename T, int DIM>
+__device__ __forceinline__ void block_counts(T mn0, T mx0, T mn1, T mx1,
+                                             T seg2eb,
+                                             typename Prec<T>::Q* nbx,
+                                             typename Prec<T>::Q* nby) {
+  using Q = typename Prec<T>::Q;
+  const T fx = dceil((mx0 - mn0) / seg2eb);
+  const T fy = dceil((mx1 - mn1) / seg2eb);
+  if (sizeof(T) == 8 && DIM == 2) {
+    *nbx = static_cast<Q>(sat_u64(static_cast<double>(fx)) + 1ull);
+  } else {
+    *nbx = static_cast<Q>(static_cast<uint32_t>(sat_u32(fx) + 1u));
+  }
+  *nby = static_cast<Q>(static_cast<uint32_t>(sat_u32(fy) + 1u));
+}
+
+// ------------------------------------------------------- workspace layout
+struct WsLayout {
+  uint64_t info, win, loc, off, meta, slots, total;
+};
+__host__ __device__ inline uint64_t align256(uint64_t v) {
+  return (v + 255) & ~255ull;
+}
+inline WsLayout ws_layout(uint32_t bits, uint64_t G) {
+  const uint64_t M = bits == 64 ? 64 : 36;
+  WsLayout w;
+  w.info = 0;
+  w.win = 256;
+  w.loc = align256(w.win + 6ull * kSampleBlocks * 8);
+  w.off = align256(w.loc + 8 * G);
+  w.meta = align256(w.off + 8 * G);
+  w.slots = align256(w.meta + M * G);
+  w.total = align256(w.slots + static_cast<uint64_t>(kSlot) * G);
+  return w;
+}
+
+struct DevInfo {  // == gpz_result, lives at workspace offset 0
+  uint64_t stream_bytes;
+  uint32_t seg_size;
+  uint32_t adjusted;
+  int32_t status;
+  uint32_t reserved;
+};
+
+// ------------------------------------------------------ seg-size kernels
+// Window b covers points [b*stride, b*stride + 256); windows that would run
+// past N are skipped by the reference kernel (src/cudaWarpCompress.cu:1230)
+// and leave the zeroed scratch in place, so they contribute 0.0.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void seg_sample_kernel(
+    const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+    uint64_t n, uint64_t stride, T* __restrict__ win) {
+  __shared__ T red[4][6];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * stride;
+  if (s + kSampleLen > n) {
+    if (t < 6) win[blockIdx.x * 6 + t] = T(0);
+    return;
+  }
+  T v[3];
+  v[0] = x[s + t];
+  v[1] = y[s + t];
+  v[2] = DIM == 3 ? z[s + t] : T(0);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T lo = wave_min(v[a]);
+    const T hi = wave_max(v[a]);
+    if (lane == 0) {
+      red[w][2 * a] = lo;
+      red[w][2 * a + 1] = hi;
+    }
+  }
+  __syncthreads();
+  if (t < 6) {
+    T r = red[0][t];
+    for (int i = 1; i < 4; ++i)
+      r = (t & 1) ? dmax(r, red[i][t]) : dmin(r, red[i][t]);
+    win[blockIdx.x * 6 + t] = r;
+  }
+}
+
+struct SegArgs {
+  uint32_t seg_in;
+  int32_t is_rel;
+  int32_t rel_small;  // (double)rel_eb <= 1e-4, src/cudaWarpCompress.cu:1517
+  int32_t sampled;    // window extrema were computed
+  double eb;
+};
+
+// src/cudaWarpCompress.cu:1488-1556 (FP32) and
+// src/cudaWarpCompressDouble.cu:1487-1553 (FP64), evaluated in T with the
+// same operation order as the reference host code.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void seg_decide_kernel(const T* __restrict__ win,
+                                                         SegArgs a,
+                                                         DevInfo* info) {
+  __shared__ T red[4][6];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (!a.sampled) {
+    if (t == 0) {
+      info->seg_size = a.seg_in;
+      info->adjusted = 0;
+      info->status = a.seg_in ? GPZ_OK : GPZ_ERR_INVALID_ARG;
+      info->stream_bytes = 0;
+    }
+    return;
+  }
+  T e[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) e[i] = (i & 1) ? -Prec<T>::lo() : Prec<T>::lo();
+  for (uint32_t b = t; b < kSampleBlocks; b += 256)
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      e[i] = (i & 1) ? dmax(e[i], win[b * 6 + i]) : dmin(e[i], win[b * 6 + i]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    e[i] = (i & 1) ? wave_max(e[i]) : wave_min(e[i]);
+    if (lane == 0) red[w][i] = e[i];
+  }
+  __syncthreads();
+  if (t != 0) return;
+  T g[6];
+  for (int i = 0; i < 6; ++i) {
+    g[i] = red[0][i];
+    for (int k = 1; k < 4; ++k)
+      g[i] = (i & 1) ? dmax(g[i], red[k][i]) : dmin(g[i], red[k][i]);
+  }
+  if (DIM == 2) { g[4] = T(0); g[5] = T(0); }
+  const T dx = g[1] - g[0], dy = g[3] - g[2], dz = g[5] - g[4];
+  const T mean = (dx + dy + dz) / T(3.0f);
+  T var;
+  if (sizeof(T) == 4 && DIM == 2)
+    var = ((dx - mean) * (dx - mean) + (dy - mean) * (dy - mean) +
+           (dz - mean) * (dz - mean)) / T(2.0f);
+  else
+    var = ((dx - mean) * (dx - mean) + (dy - mean) * (dy - mean) +
+           (dz - mean) * (dz - mean)) / T(3.0f);
+  const T ratio = sqrt(var) / mean;
+  T max_range = dmax(dx, dy);
+  max_range = dmax(max_range, dz);
+  uint32_t seg = a.seg_in;
+  if (seg == 0 && a.is_rel) {
+    if (ratio < T(0.1f)) seg = a.rel_small ? 32u : 2u;
+    else seg = a.rel_small ? 4u : 1u;
+  }
+  int32_t status = GPZ_OK;
+  uint32_t adjusted = 0;
+  if (sizeof(T) == 4) {
+    const T eb = static_cast<T>(a.eb);
+    auto blocks = [&](uint32_t s) -> uint64_t {
+      const uint64_t mb =
+          sat_u64(static_cast<double>(dceil(max_range / (static_cast<T>(2u * s) * eb)))) + 1ull;
+      return DIM == 3 ? mb * mb * mb : mb * mb;
+    };
+    if (blocks(seg) > 0xFFFFFFFFull) {
+      uint32_t tmp = seg;
+      for (int i = 0; i < 11; ++i) {
+        tmp *= 2;
+        if (blocks(tmp) <= 0xFFFFFFFFull) break;
+      }
+      if (tmp > 1024) status = GPZ_ERR_SEG_TOO_LARGE;
+      seg = tmp;
+      adjusted = 1;
+    }
+  }
+  if (seg == 0 && status == GPZ_OK) status = GPZ_ERR_INVALID_ARG;
+  info->seg_size = seg;
+  info->adjusted = adjusted;
+  info->status = status;
+  info->stream_bytes = 0;
+}
+
+// ------------------------------------------------------------- encoder
+struct EncodeArgs {
+  const void* x;
+  const void* y;
+  const void* z;
+  uint64_t n, G;
+  double eb;
+  const DevInfo* info;
+  uint8_t* meta;    // [G * kMeta]
+  uint64_t* loc;    // [G]
+  uint8_t* slots;   // [G * kSlot]
+};
+
+__device__ __forceinline__ uint32_t pad_word(uint32_t p) {
+  return p + ((p >> 4) << 2);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
+  using Q = typename Prec<T>::Q;
+  // s_sort holds keys | values during the sort and is reused as the payload
+  // staging buffer afterwards (10240 >= kSlot + 16 bytes).
+  __shared__ __align__(16) uint32_t s_sort[2 * kSortWords];
+  __shared__ __align__(16) uint16_t s_cnt[16 * 64];
+  __shared__ __align__(16) uint16_t s_rstart[kGroup + 16];
+  __shared__ __align__(16) uint32_t s_rdelta[kGroup];
+
+  const uint32_t lane = threadIdx.x;
+  const uint64_t c = blockIdx.x;
+  if (a.info->status != GPZ_OK) return;
+  const uint32_t seg = a.info->seg_size;
+  const T eb = static_cast<T>(a.eb);
+  const uint64_t base = c * kGroup;
+  const uint32_t nvalid =
+      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
+  const bool last = (c == a.G - 1);
+  const T* X = static_cast<const T*>(a.x);
+  const T* Y = static_cast<const T*>(a.y);
+  const T* Z = static_cast<const T*>(a.z);
+
+  // ---- stage 1: load in blocked sort order, group extrema
+  T px[16], py[16], pz[16];
+  T mn[3] = {Prec<T>::lo(), Prec<T>::lo(), Prec<T>::lo()};
+  T mx[3] = {-Prec<T>::lo(), -Prec<T>::lo(), -Prec<T>::lo()};
+  const uint32_t jbase = 512u * (lane & 1u) + (lane >> 1);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t j = jbase + 32u * k;
+    px[k] = py[k] = pz[k] = T(0);
+    if (j < nvalid) {
+      px[k] = X[base + j];
+      py[k] = Y[base + j];
+      mn[0] = dmin(mn[0], px[k]);
+      mx[0] = dmax(mx[0], px[k]);
+      mn[1] = dmin(mn[1], py[k]);
+      mx[1] = dmax(mx[1], py[k]);
+      if (DIM == 3) {
+        pz[k] = Z[base + j];
+        mn[2] = dmin(mn[2], pz[k]);
+        mx[2] = dmax(mx[2], pz[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    mn[d] = wave_min(mn[d]);
+    mx[d] = wave_max(mx[d]);
+  }
+  if (DIM == 2) { mn[2] = T(0); mx[2] = T(0); }
+
+  // ---- stage 2: quantise, block id, relative position
+  const T inv = T(1) / (T(2) * eb);
+  const T seg2eb = static_cast<T>(2u * seg) * eb;
+  Q nbx, nby;
+  block_counts<T, DIM>(mn[0], mx[0], mn[1], mx[1], seg2eb, &nbx, &nby);
+  const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
+  const Q m = static_cast<Q>(seg - 1u);
+  const Q sq = static_cast<Q>(seg);
+  const T nmi0 = -mn[0] * inv, nmi1 = -mn[1] * inv, nmi2 = -mn[2] * inv;
+  uint32_t key[16], val[16];
+  uint32_t bb = 0, rbl = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t j = jbase + 32u * k;
+    if (j < nvalid) {
+      Q qx, qy, qz = 0;
+      if (sizeof(T) == 4) {
+        qx = static_cast<Q>(sat_u32(dfma(px[k], inv, nmi0)));
+        qy = static_cast<Q>(sat_u32(dfma(py[k], inv, nmi1)));
+        if (DIM == 3) qz = static_cast<Q>(sat_u32(dfma(pz[k], inv, nmi2)));
+      } else {
+        qx = static_cast<Q>(sat_u64(static_cast<double>(dfma(px[k], inv, nmi0))));
+        qy = static_cast<Q>(sat_u64(static_cast<double>(dfma(py[k], inv, nmi1))));
+        if (DIM == 3)
+          qz = static_cast<Q>(sat_u64(static_cast<double>(dfma(pz[k], inv, nmi2))));
+      }
+      Q b = shr(qx, sh) + shr(qy, sh) * nbx;
+      Q r = (qx & m) + (qy & m) * sq;
+      if (DIM == 3) {
+        b += shr(qz, sh) * nbx * nby;
+        r += (qz & m) * sq * sq;
+      }
+      key[k] = static_cast<uint32_t>(b & Q(0x7FFFFFFF));
+      val[k] = static_cast<uint32_t>(r & Q(0x7FFFFFFF));
+      bb = max(bb, eff_bits(key[k]));
+      rbl = max(rbl, eff_bits(val[k]));
+    } else {
+      key[k] = 0xFFFFFFFFu;
+      val[k] = 0xFFFFFFFFu;
+      bb = max(bb, 1u);
+      rbl = max(rbl, 1u);
+    }
+  }
+  const uint32_t bidbits = wave_max_u32(bb);
+  const uint32_t rb = wave_max_u32(rbl);
+
+  // ---- stage 3: stable LSD radix sort, 4-bit digits, over [0, sortbits)
+  // Counters are private per lane (column = lane), so counting is serial
+  // and conflict free; an exclusive scan in (digit, lane) order then gives
+  // each item its stable rank.
+  const uint32_t sortbits = bidbits + (last ? 1u : 0u);
+  uint32_t* s_key = s_sort;
+  uint32_t* s_val = s_sort + kSortWords;
+  for (uint32_t shift = 0; shift < sortbits; shift += 4) {
+    reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(0, 0, 0, 0);
+    reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    uint32_t dig[16], lrank[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      dig[k] = (key[k] >> shift) & 15u;
+      const uint32_t idx = dig[k] * 64u + lane;
+      const uint32_t o = s_cnt[idx];
+      s_cnt[idx] = static_cast<uint16_t>(o + 1u);
+      lrank[k] = o;
+    }
+    __syncthreads();
+    {
+      uint4 c0 = reinterpret_cast<uint4*>(s_cnt)[2 * lane];
+      uint4 c1 = reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1];
+      uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      uint32_t run = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t lo = w[i] & 0xFFFFu, hi = w[i] >> 16;
+        const uint32_t elo = run;
+        run += lo;
+        const uint32_t ehi = run;
+        run += hi;
+        w[i] = elo | (ehi << 16);
+      }
+      const uint32_t incl = wave_incl_scan(run, lane);
+      const uint32_t add = incl - run;
+      const uint32_t add2 = add | (add << 16);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) w[i] += add2;
+      reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(w[0], w[1], w[2], w[3]);
+      reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t r = s_cnt[dig[k] * 64u + lane] + lrank[k];
+      const uint32_t wi = pad_word(r);
+      s_key[wi] = key[k];
+      s_val[wi] = val[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q4 = 0; q4 < 4; ++q4) {
+      const uint4 kk = reinterpret_cast<const uint4*>(s_key)[5 * lane + q4];
+      const uint4 vv = reinterpret_cast<const uint4*>(s_val)[5 * lane + q4];
+      key[4 * q4 + 0] = kk.x; key[4 * q4 + 1] = kk.y;
+      key[4 * q4 + 2] = kk.z; key[4 * q4 + 3] = kk.w;
+      val[4 * q4 + 0] = vv.x; val[4 * q4 + 1] = vv.y;
+      val[4 * q4 + 2] = vv.z; val[4 * q4 + 3] = vv.w;
+    }
+    __syncthreads();
+  }
+
+  // ---- stage 4a: runs of equal block id (lane l holds ranks 16l..16l+15)
+  const uint32_t prev0 =
+      static_cast<uint32_t>(__shfl_up(static_cast<int>(key[15]), 1));
+  uint32_t fl = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t r = 16u * lane + k;
+    const uint32_t prev = k ? key[k - 1] : prev0;
+    const bool f = (r < nvalid) && (r == 0 || key[k] != prev);
+    fl |= static_cast<uint32_t>(f) << k;
+  }
+  const uint32_t nf = __popc(fl);
+  const uint32_t fincl = wave_incl_scan(nf, lane);
+  const uint32_t U = static_cast<uint32_t>(__shfl(static_cast<int>(fincl), 63));
+  {
+    uint32_t idx = fincl - nf;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if ((fl >> k) & 1u) {
+        const uint32_t r = 16u * lane + k;
+        const uint32_t prev = k ? key[k - 1] : prev0;
+        s_rstart[idx] = static_cast<uint16_t>(r);
+        s_rdelta[idx] = r == 0 ? 0u : key[k] - prev;
+        ++idx;
+      }
+    }
+    if (lane == 0) s_rstart[U] = static_cast<uint16_t>(nvalid);
+  }
+  const uint32_t outlier =
+      static_cast<uint32_t>(__shfl(static_cast<int>(key[0]), 0));
+  __syncthreads();
+  uint32_t del[16], cnt[16];
+  uint32_t dbl = 0, cbl = 0;
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const uint32_t i = 16u * lane + t;
+    del[t] = 0;
+    cnt[t] = 0;
+    if (i < U) {
+      del[t] = s_rdelta[i];
+      cnt[t] = static_cast<uint32_t>(s_rstart[i + 1]) - s_rstart[i];
+      dbl = max(dbl, eff_bits(del[t]));
+      cbl = max(cbl, eff_bits(cnt[t]));
+    }
+  }
+  const uint32_t db = wave_max_u32(dbl);
+  const uint32_t cb = wave_max_u32(cbl);
+  uint32_t D, C, R;
+  section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
+  const uint32_t loc = D + C + R;
+  const uint32_t B = (U + 7u) >> 3;
+
+  // ---- stage 4b: bit planes into the LDS staging buffer
+  uint8_t* stg = reinterpret_cast<uint8_t*>(s_sort);
+  const uint32_t loc16 = (loc + 15u) & ~15u;
+  for (uint32_t o = 16u * lane; o < loc16; o += 1024u)
+    *reinterpret_cast<uint4*>(stg + o) = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  for (uint32_t p = 0; p < db; ++p) {
+    const uint32_t b = db - 1u - p;
+    uint32_t b0 = 0, b1 = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      b0 |= ((del[u] >> b) & 1u) << (7 - u);
+      b1 |= ((del[8 + u] >> b) & 1u) << (7 - u);
+    }
+    if (2u * lane < B) stg[p * B + 2u * lane] = static_cast<uint8_t>(b0);
+    if (2u * lane + 1u < B) stg[p * B + 2u * lane + 1u] = static_cast<uint8_t>(b1);
+  }
+  for (uint32_t p = 0; p < cb; ++p) {
+    const uint32_t b = cb - 1u - p;
+    uint32_t b0 = 0, b1 = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      b0 |= ((cnt[u] >> b) & 1u) << (7 - u);
+      b1 |= ((cnt[8 + u] >> b) & 1u) << (7 - u);
+    }
+    if (2u * lane < B) stg[D + p * B + 2u * lane] = static_cast<uint8_t>(b0);
+    if (2u * lane + 1u < B) stg[D + p * B + 2u * lane + 1u] = static_cast<uint8_t>(b1);
+  }
+  for (uint32_t p = 0; p < rb; ++p) {
+    const uint32_t b = rb - 1u - p;
+    uint32_t wv = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) wv |= ((val[k] >> b) & 1u) << k;
+    *reinterpret_cast<uint16_t*>(stg + D + C + p * 128u + 2u * lane) =
+        static_cast<uint16_t>(wv);
+  }
+  __syncthreads();
+
+  // ---- meta, size, payload
+  {
+    constexpr uint32_t M = Prec<T>::kMeta;
+    uint32_t word = 0;
+    if (sizeof(T) == 4) {
+      if (lane < 6) {
+        const float f = static_cast<float>(lane < 3 ? mn[lane] : mx[lane - 3]);
+        word = __float_as_uint(f);
+      } else if (lane == 6) {
+        word = outlier;
+      } else if (lane == 7) {
+        word = db | (cb << 16);
+      } else if (lane == 8) {
+        word = rb | (U << 16);
+      }
+    } else {
+      if (lane < 12) {
+        const uint32_t e = lane >> 1;
+        const double d = static_cast<double>(e < 3 ? mn[e] : mx[e - 3]);
+        const uint64_t u = static_cast<uint64_t>(__double_as_longlong(d));
+        word = (lane & 1) ? static_cast<uint32_t>(u >> 32) : static_cast<uint32_t>(u);
+      } else if (lane == 12) {
+        word = outlier;
+      } else if (lane == 13) {
+        word = 0;
+      } else if (lane == 14) {
+        word = db | (cb << 16);
+      } else if (lane == 15) {
+        word = rb | (U << 16);
+      }
+    }
+    if (lane < M / 4)
+      reinterpret_cast<uint32_t*>(a.meta + c * M)[lane] = word;
+  }
+  if (lane == 0) a.loc[c] = loc;
+  uint8_t* slot = a.slots + c * static_cast<uint64_t>(kSlot);
+  for (uint32_t o = 16u * lane; o < loc16; o += 1024u)
+    *reinterpret_cast<uint4*>(slot + o) = *reinterpret_cast<const uint4*>(stg + o);
+}
+
+// ------------------------------------------------- offsets scan + header
+struct ScanArgs {
+  const uint64_t* loc;
+  uint64_t* off;
+  uint8_t* stream;
+  DevInfo* info;
+  uint64_t n, G;
+  uint32_t dim;
+  double eb;
+};
+
+template <typename T>
+__global__ __launch_bounds__(1024) void scan_kernel(ScanArgs a) {
+  __shared__ uint64_t wsum[16];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  if (a.info->status != GPZ_OK) return;
+  const uint64_t per = (a.G + 1023) / 1024;
+  const uint64_t b0 = t * per;
+  const uint64_t b1 = b0 + per < a.G ? b0 + per : a.G;
+  uint64_t s = 0;
+  for (uint64_t i = b0; i < b1; ++i) s += a.loc[i];
+  const uint64_t incl = wave_incl_scan(s, lane);
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint64_t wbase = 0;
+  for (uint32_t i = 0; i < w; ++i) wbase += wsum[i];
+  uint64_t run = wbase + incl - s;
+  uint64_t* soff = reinterpret_cast<uint64_t*>(a.stream + P);
+  for (uint64_t i = b0; i < b1; ++i) {
+    a.off[i] = run;
+    soff[i] = run;
+    run += a.loc[i];
+  }
+  if (t == 1023) {
+    uint64_t total = 0;
+    for (int i = 0; i < 16; ++i) total += wsum[i];
+    a.info->stream_bytes = P + (8ull + M) * a.G + total;
+  }
+  if (t == 0) {
+    uint32_t* h = reinterpret_cast<uint32_t*>(a.stream);
+    h[0] = a.dim;
+    h[1] = static_cast<uint32_t>(a.n);
+    h[2] = static_cast<uint32_t>(a.n >> 32);
+    h[3] = sizeof(T) * 8;
+    if (sizeof(T) == 4) {
+      h[4] = __float_as_uint(static_cast<float>(a.eb));
+      h[5] = a.info->seg_size;
+    } else {
+      const uint64_t u = static_cast<uint64_t>(__double_as_longlong(a.eb));
+      h[4] = static_cast<uint32_t>(u);
+      h[5] = static_cast<uint32_t>(u >> 32);
+      h[6] = a.info->seg_size;
+      h[7] = 32;   // parameter_size, src/cudaWarpCompressDouble.cu:1674-1680
+    }
+  }
+}
+
+struct CompactArgs {
+  const uint64_t* loc;
+  const uint64_t* off;
+  const uint8_t* meta;
+  const uint8_t* slots;
+  uint8_t* stream;
+  const DevInfo* info;
+  uint64_t G;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
+  constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  if (a.info->status != GPZ_OK) return;
+  const uint64_t c = blockIdx.x;
+  const uint32_t t = threadIdx.x;
+  if (t < M / 4)
+    reinterpret_cast<uint32_t*>(a.stream + P + 8 * a.G + M * c)[t] =
+        reinterpret_cast<const uint32_t*>(a.meta + M * c)[t];
+  const uint32_t words = static_cast<uint32_t>(a.loc[c]) >> 2;
+  const uint32_t* src =
+      reinterpret_cast<const uint32_t*>(a.slots + c * static_cast<uint64_t>(kSlot));
+  uint32_t* dst = reinterpret_cast<uint32_t*>(a.stream + P + (8ull + M) * a.G +
+                                              a.off[c]);
+  for (uint32_t i = t; i < words; i += 256) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------- decoder
+struct DecodeArgs {
+  const uint8_t* stream;
+  uint64_t n, G;
+  double eb;
+  uint32_t seg;
+  void* x;
+  void* y;
+  void* z;
+};
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
+  using Q = typename Prec<T>::Q;
+  constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  __shared__ __align__(16) Q s_bid[kGroup];
+  __shared__ __align__(16) uint8_t s_start[kGroup];
+
+  const uint32_t lane = threadIdx.x;
+  const uint64_t c = blockIdx.x;
+  const uint8_t* meta = a.stream + P + 8 * a.G + M * c;
+  T mm[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) mm[i] = reinterpret_cast<const T*>(meta)[i];
+  const Q outlier = *reinterpret_cast<const Q*>(meta + 6 * sizeof(T));
+  const uint16_t* u16 = reinterpret_cast<const uint16_t*>(meta + 6 * sizeof(T) + sizeof(Q));
+  const uint32_t db = u16[0], cb = u16[1], rb = u16[2];
+  uint32_t U = u16[3];
+  if (U > kGroup) U = kGroup;
+  const uint64_t off = reinterpret_cast<const uint64_t*>(a.stream + P)[c];
+  const uint8_t* pd = a.stream + P + (8ull + M) * a.G + off;
+  uint32_t D, C, R;
+  // FP64-2D streams carry a 4-byte-padded count section (encoder layout).
+  section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
+  const uint8_t* pc = pd + D;
+  const uint8_t* pr = pd + D + C;
+  const uint32_t B = (U + 7u) >> 3;
+  const T eb = static_cast<T>(a.eb);
+  const uint32_t seg = a.seg;
+  const T seg2eb = static_cast<T>(2u * seg) * eb;
+  Q nbx, nby;
+  block_counts<T, DIM>(mm[0], mm[3], mm[1], mm[4], seg2eb, &nbx, &nby);
+  const uint64_t base = c * kGroup;
+  const uint32_t nvalid =
+      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
+
+  // runs 16l..16l+15: deltas and counts from MSB-first planes
+  Q del[16];
+  uint32_t cnt[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) { del[t] = 0; cnt[t] = 0; }
+  for (uint32_t p = 0; p < db; ++p) {
+    const uint32_t b = db - 1u - p;
+    const uint32_t b0 = 2u * lane < B ? pd[p * B + 2u * lane] : 0u;
+    const uint32_t b1 = 2u * lane + 1u < B ? pd[p * B + 2u * lane + 1u] : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      del[u] |= static_cast<Q>((b0 >> (7 - u)) & 1u) << b;
+      del[8 + u] |= static_cast<Q>((b1 >> (7 - u)) & 1u) << b;
+    }
+  }
+  for (uint32_t p = 0; p < cb; ++p) {
+    const uint32_t b = cb - 1u - p;
+    const uint32_t b0 = 2u * lane < B ? pc[p * B + 2u * lane] : 0u;
+    const uint32_t b1 = 2u * lane + 1u < B ? pc[p * B + 2u * lane + 1u] : 0u;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      cnt[u] |= ((b0 >> (7 - u)) & 1u) << b;
+      cnt[8 + u] |= ((b1 >> (7 - u)) & 1u) << b;
+    }
+  }
+  // block id per run = outlier + prefix of deltas (delta of run 0 ignored,
+  // src/cudaWarpCompress.cu:930-935); run end = prefix of counts.
+  Q bsum = 0;
+  uint32_t csum = 0;
+  Q bid[16];
+  uint32_t cend[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const uint32_t i = 16u * lane + t;
+    const Q d = (i == 0) ? outlier : (i < U ? del[t] : Q(0));
+    const uint32_t cn = i < U ? cnt[t] : 0u;
+    bsum += d;
+    csum += cn;
+    bid[t] = bsum;
+    cend[t] = csum;
+  }
+  const Q bincl = wave_incl_scan(static_cast<uint64_t>(bsum), lane);
+  const uint32_t cincl = wave_incl_scan(csum, lane);
+  const Q bpre = static_cast<Q>(bincl - bsum);
+  const uint32_t cpre = cincl - csum;
+  reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const uint32_t i = 16u * lane + t;
+    if (i < U) {
+      s_bid[i] = static_cast<Q>(bid[t] + bpre);
+      const uint32_t st = cend[t] + cpre - cnt[t];
+      if (st < kGroup) s_start[st] = 1;
+    }
+  }
+  __syncthreads();
+  // rank -> run index by a prefix count of run starts
+  const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
+  const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
+  uint32_t rs = 0;
+  uint32_t runof[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    rs += (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+    runof[k] = rs;
+  }
+  const uint32_t rincl = wave_incl_scan(rs, lane);
+  const uint32_t rpre = rincl - rs;
+  // relative positions from LSB-first planes
+  Q rel[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) rel[k] = 0;
+  for (uint32_t p = 0; p < rb; ++p) {
+    const uint32_t b = rb - 1u - p;
+    const uint32_t wv = *reinterpret_cast<const uint16_t*>(pr + p * 128u + 2u * lane);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) rel[k] |= static_cast<Q>((wv >> k) & 1u) << b;
+  }
+  const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
+  const Q m = static_cast<Q>(seg - 1u);
+  const Q sq = static_cast<Q>(seg);
+  const Q nxy = static_cast<Q>(nbx * nby);
+  T* X = static_cast<T*>(a.x);
+  T* Y = static_cast<T*>(a.y);
+  T* Z = static_cast<T*>(a.z);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t r = 16u * lane + k;
+    if (r < nvalid) {
+      uint32_t run = runof[k] + rpre;
+      run = run ? run - 1u : 0u;
+      if (run >= U) run = U ? U - 1u : 0u;
+      const Q b = s_bid[run];
+      const Q bx = urem<Q>(b, nbx);
+      const Q by = urem<Q>(udiv<Q>(b, nbx), nby);
+      const Q qx = bx * sq + (rel[k] & m);
+      const Q qy = by * sq + (shr(rel[k], sh) & m);
+      X[base + r] = dfma(static_cast<T>((qx << 1) | 1u), eb, mm[0]);
+      Y[base + r] = dfma(static_cast<T>((qy << 1) | 1u), eb, mm[1]);
+      if (DIM == 3) {
+        const Q bz = udiv<Q>(b, nxy);
+        const Q qz = bz * sq + shr(rel[k], 2u * sh);
+        Z[base + r] = dfma(static_cast<T>((qz << 1) | 1u), eb, mm[2]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------- min / max
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ x,
+                                                     const T* __restrict__ y,
+                                                     const T* __restrict__ z,
+                                                     uint64_t n, T* part) {
+  __shared__ T red[4][6];
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  T e[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) e[i] = (i & 1) ? -Prec<T>::lo() : Prec<T>::lo();
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + t; i < n; i += stride) {
+    const T vx = x[i], vy = y[i];
+    e[0] = dmin(e[0], vx); e[1] = dmax(e[1], vx);
+    e[2] = dmin(e[2], vy); e[3] = dmax(e[3], vy);
+    if (DIM == 3) {
+      const T vz = z[i];
+      e[4] = dmin(e[4], vz); e[5] = dmax(e[5], vz);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    e[i] = (i & 1) ? wave_max(e[i]) : wave_min(e[i]);
+    if (lane == 0) red[w][i] = e[i];
+  }
+  __syncthreads();
+  if (t < 6) {
+    T r = red[0][t];
+    for (int k = 1; k < 4; ++k) r = (t & 1) ? dmax(r, red[k][t]) : dmin(r, red[k][t]);
+    part[blockIdx.x * 6 + t] = r;
+  }
+}
+
+// --------------------------------------------------------------- host
+template <typename T> struct HostPrec;
+template <> struct HostPrec<float> { static constexpr uint32_t P = 24, M = 36; };
+template <> struct HostPrec<double> { static constexpr uint32_t P = 32, M = 64; };
+
+inline uint64_t groups_of(uint64_t n) { return (n + kGroup - 1) / kGroup; }
+
+bool params_ok(const gpz_params* p) {
+  if (!p) return false;
+  if (p->dim != 2 && p->dim != 3) return false;
+  if (p->float_bits != 32 && p->float_bits != 64) return false;
+  if (p->n == 0 || groups_of(p->n) > 0x7FFFFFFFull) return false;
+  if (p->mode != GPZ_MODE_ABS && p->mode != GPZ_MODE_REL) return false;
+  if (!(p->eb >= 0.0) || p->eb > 1.0e300) return false;
+  // abs mode without an explicit segment size leaves seg = 0 in the
+  // reference (src/cudaWarpCompress.cu:1513-1531), which breaks the bound;
+  // it is rejected here.
+  if (p->mode == GPZ_MODE_ABS && p->seg_size == 0) return false;
+  return true;
+}
+
+template <typename T, int DIM>
+int compress_impl(const gpz_params* p, const void* dx, const void* dy,
+                  const void* dz, void* dstream, void* dws,
+                  hipStream_t s) {
+  const uint64_t n = p->n, G = groups_of(n);
+  const WsLayout L = ws_layout(sizeof(T) * 8, G);
+  uint8_t* ws = static_cast<uint8_t*>(dws);
+  DevInfo* info = reinterpret_cast<DevInfo*>(ws + L.info);
+  T* win = reinterpret_cast<T*>(ws + L.win);
+  // FP32 always runs the sampler (its block-count guard can double any seg,
+  // src/cudaWarpCompress.cu:1532-1554); FP64 only needs it to choose one.
+  const bool sample = sizeof(T) == 4 || (p->seg_size == 0 && p->mode == GPZ_MODE_REL);
+  if (sample) {
+    const uint64_t stride = n - kSampleLen;  // u64 wrap for n < 256, as the reference
+    uint64_t st = stride / kSampleBlocks;
+    if (st < 1) st = 1;
+    hipLaunchKernelGGL((seg_sample_kernel<T, DIM>), dim3(kSampleBlocks), dim3(256), 0, s,
+                       static_cast<const T*>(dx), static_cast<const T*>(dy),
+                       static_cast<const T*>(dz), n, st, win);
+  }
+  SegArgs sa;
+  sa.seg_in = p->seg_size;
+  sa.is_rel = p->mode == GPZ_MODE_REL;
+  const double rel = sizeof(T) == 4 ? static_cast<double>(static_cast<float>(p->rel_eb))
+                                    : p->rel_eb;
+  sa.rel_small = rel <= 1e-4;
+  sa.sampled = sample;
+  sa.eb = sizeof(T) == 4 ? static_cast<double>(static_cast<float>(p->eb)) : p->eb;
+  hipLaunchKernelGGL((seg_decide_kernel<T, DIM>), dim3(1), dim3(256), 0, s, win, sa, info);
+
+  EncodeArgs ea;
+  ea.x = dx; ea.y = dy; ea.z = dz;
+  ea.n = n; ea.G = G; ea.eb = sa.eb;
+  ea.info = info;
+  ea.meta = ws + L.meta;
+  ea.loc = reinterpret_cast<uint64_t*>(ws + L.loc);
+  ea.slots = ws + L.slots;
+  hipLaunchKernelGGL((encode_kernel<T, DIM>), dim3(static_cast<uint32_t>(G)), dim3(64), 0, s, ea);
+
+  ScanArgs sc;
+  sc.loc = ea.loc;
+  sc.off = reinterpret_cast<uint64_t*>(ws + L.off);
+  sc.stream = static_cast<uint8_t*>(dstream);
+  sc.info = info;
+  sc.n = n; sc.G = G; sc.dim = DIM; sc.eb = sa.eb;
+  hipLaunchKernelGGL((scan_kernel<T>), dim3(1), dim3(1024), 0, s, sc);
+
+  CompactArgs ca;
+  ca.loc = ea.loc;
+  ca.off = sc.off;
+  ca.meta = ea.meta;
+  ca.slots = ea.slots;
+  ca.stream = sc.stream;
+  ca.info = info;
+  ca.G = G;
+  hipLaunchKernelGGL((compact_kernel<T>), dim3(static_cast<uint32_t>(G)), dim3(256), 0, s, ca);
+  return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
+}
+
+template <typename T, int DIM>
+int decompress_impl(const gpz_header* h, const void* dstream, void* dx,
+                    void* dy, void* dz, hipStream_t s) {
+  DecodeArgs da;
+  da.stream = static_cast<const uint8_t*>(dstream);
+  da.n = h->n;
+  da.G = h->groups;
+  da.eb = h->eb;
+  da.seg = h->seg_size;
+  da.x = dx; da.y = dy; da.z = dz;
+  hipLaunchKernelGGL((decode_kernel<T, DIM>), dim3(static_cast<uint32_t>(h->groups)),
+                     dim3(64), 0, s, da);
+  return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
+}
+
+template <typename T, int DIM>
+int minmax_impl(uint64_t n, const void* dx, const void* dy, const void* dz,
+                void* dws, void* mm, hipStream_t s) {
+  T* part = static_cast<T*>(dws);
+  hipLaunchKernelGGL((minmax_kernel<T, DIM>), dim3(kMinMaxBlocks), dim3(256), 0, s,
+                     static_cast<const T*>(dx), static_cast<const T*>(dy),
+                     static_cast<const T*>(dz), n, part);
+  if (hipGetLastError() != hipSuccess) return GPZ_ERR_HIP;
+  T h[kMinMaxBlocks * 6];
+  if (hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPZ_ERR_HIP;
+  // Same result as the reference's serial std::min/std::max sweep for
+  // finite data (min/max are exact); the z extent is 0 in 2D.
+  T r[6];
+  for (int i = 0; i < 6; ++i) r[i] = h[i];
+  for (uint32_t b = 1; b < kMinMaxBlocks; ++b)
+    for (int i = 0; i < 6; ++i)
+      r[i] = (i & 1) ? (h[b * 6 + i] > r[i] ? h[b * 6 + i] : r[i])
+                     : (h[b * 6 + i] < r[i] ? h[b * 6 + i] : r[i]);
+  const T out[6] = {r[0], r[2], DIM == 3 ? r[4] : T(0),
+                    r[1], r[3], DIM == 3 ? r[5] : T(0)};
+  memcpy(mm, out, sizeof(out));
+  return GPZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpz_version(void) { return 100; }
+
+uint64_t gpz_stream_bound(const gpz_params* p) {
+  if (!p || p->n == 0) return 0;
+  const uint64_t G = groups_of(p->n);
+  const uint64_t P = p->float_bits == 64 ? 32 : 24;
+  const uint64_t M = p->float_bits == 64 ? 64 : 36;
+  return P + (8 + M) * G + static_cast<uint64_t>(kSlot) * G;
+}
+
+uint64_t gpz_compress_workspace_bytes(const gpz_params* p) {
+  if (!p || p->n == 0) return 0;
+  return ws_layout(p->float_bits == 64 ? 64 : 32, groups_of(p->n)).total;
+}
+
+int gpz_compress(const gpz_params* p, const void* d_x, const void* d_y,
+                 const void* d_z, void* d_stream, uint64_t cap, void* d_ws,
+                 uint64_t ws_bytes, gpz_stream_t stream) {
+  if (!params_ok(p) || !d_x || !d_y || (p->dim == 3 && !d_z) || !d_stream)
+    return GPZ_ERR_INVALID_ARG;
+  if (!d_ws || ws_bytes < gpz_compress_workspace_bytes(p)) return GPZ_ERR_WORKSPACE;
+  if (cap < gpz_stream_bound(p)) return GPZ_ERR_CAPACITY;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (p->float_bits == 32)
+    return p->dim == 3 ? compress_impl<float, 3>(p, d_x, d_y, d_z, d_stream, d_ws, s)
+                       : compress_impl<float, 2>(p, d_x, d_y, d_z, d_stream, d_ws, s);
+  return p->dim == 3 ? compress_impl<double, 3>(p, d_x, d_y, d_z, d_stream, d_ws, s)
+                     : compress_impl<double, 2>(p, d_x, d_y, d_z, d_stream, d_ws, s);
+}
+
+int gpz_compress_result(const void* d_ws, gpz_result* out, gpz_stream_t stream) {
+  if (!d_ws || !out) return GPZ_ERR_INVALID_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(out, d_ws, sizeof(gpz_result), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPZ_ERR_HIP;
+  return GPZ_OK;
+}
+
+int gpz_parse_header(const void* h_stream, uint64_t bytes, gpz_header* out) {
+  if (!h_stream || !out || bytes < 24) return GPZ_ERR_INVALID_ARG;
+  const uint8_t* b = static_cast<const uint8_t*>(h_stream);
+  gpz_header h;
+  memset(&h, 0, sizeof(h));
+  memcpy(&h.dim, b, 4);
+  memcpy(&h.n, b + 4, 8);
+  memcpy(&h.float_bits, b + 12, 4);
+  if (h.float_bits == 32) {
+    float e;
+    memcpy(&e, b + 16, 4);
+    h.eb = e;
+    memcpy(&h.seg_size, b + 20, 4);
+    h.header_bytes = 24;
+  } else if (h.float_bits == 64) {
+    if (bytes < 32) return GPZ_ERR_CORRUPT;
+    memcpy(&h.eb, b + 16, 8);
+    memcpy(&h.seg_size, b + 24, 4);
+    h.header_bytes = 32;
+  } else {
+    return GPZ_ERR_CORRUPT;
+  }
+  if ((h.dim != 2 && h.dim != 3) || h.n == 0 || h.seg_size == 0 ||
+      groups_of(h.n) > 0x7FFFFFFFull)
+    return GPZ_ERR_CORRUPT;
+  h.groups = groups_of(h.n);
+  h.min_stream_bytes = h.header_bytes + (8ull + (h.float_bits == 64 ? 64 : 36)) * h.groups;
+  *out = h;
+  return GPZ_OK;
+}
+
+int gpz_read_header(const void* d_stream, gpz_header* out, gpz_stream_t stream) {
+  if (!d_stream || !out) return GPZ_ERR_INVALID_ARG;
+  uint8_t h[32];
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(h, d_stream, 32, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPZ_ERR_HIP;
+  return gpz_parse_header(h, 32, out);
+}
+
+int gpz_decompress(const gpz_header* h, const void* d_stream,
+                   uint64_t stream_bytes, void* d_x, void* d_y, void* d_z,
+                   gpz_stream_t stream) {
+  if (!h || !d_stream || !d_x || !d_y || (h->dim == 3 && !d_z))
+    return GPZ_ERR_INVALID_ARG;
+  if ((h->dim != 2 && h->dim != 3) || h->n == 0 || h->seg_size == 0 ||
+      h->groups != groups_of(h->n) || stream_bytes < h->min_stream_bytes)
+    return GPZ_ERR_CORRUPT;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (h->float_bits == 32)
+    return h->dim == 3 ? decompress_impl<float, 3>(h, d_stream, d_x, d_y, d_z, s)
+                       : decompress_impl<float, 2>(h, d_stream, d_x, d_y, d_z, s);
+  if (h->float_bits == 64)
+    return h->dim == 3 ? decompress_impl<double, 3>(h, d_stream, d_x, d_y, d_z, s)
+                       : decompress_impl<double, 2>(h, d_stream, d_x, d_y, d_z, s);
+  return GPZ_ERR_CORRUPT;
+}
+
+uint64_t gpz_min_max_workspace_bytes(void) { return 6ull * kMinMaxBlocks * 8; }
+
+int gpz_min_max(uint32_t dim, uint32_t float_bits, uint64_t n,
+                const void* d_x, const void* d_y, const void* d_z, void* d_ws,
+                void* mm, gpz_stream_t stream) {
+  if ((dim != 2 && dim != 3) || n == 0 || !d_x || !d_y || (dim == 3 && !d_z) ||
+      !d_ws || !mm)
+    return GPZ_ERR_INVALID_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (float_bits == 32)
+    return dim == 3 ? minmax_impl<float, 3>(n, d_x, d_y, d_z, d_ws, mm, s)
+                    : minmax_impl<float, 2>(n, d_x, d_y, d_z, d_ws, mm, s);
+  if (float_bits == 64)
+    return dim == 3 ? minmax_impl<double, 3>(n, d_x, d_y, d_z, d_ws, mm, s)
+                    : minmax_impl<double, 2>(n, d_x, d_y, d_z, d_ws, mm, s);
+  return GPZ_ERR_INVALID_ARG;
+}
+
+double gpz_rel_to_abs(uint32_t float_bits, double e, const void* mm) {
+  if (!mm) return 0.0;
+  if (float_bits == 32) {
+    const float* v = static_cast<const float*>(mm);
+    float r = fmaxf(v[3] - v[0], v[4] - v[1]);
+    r = fmaxf(r, v[5] - v[2]);
+    return static_cast<double>(r * static_cast<float>(e));
+  }
+  const double* v = static_cast<const double*>(mm);
+  double r = fmax(v[3] - v[0], v[4] - v[1]);
+  r = fmax(r, v[5] - v[2]);
+  return r * e;
+}
+
+}  // extern "C"
