@@ -730,6 +730,7 @@ __global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
 // ------------------------------------------------------------- decoder
 struct DecodeArgs {
   const uint8_t* stream;
+  uint64_t stream_bytes;
   uint64_t n, G;
   double eb;
   uint32_t seg;
@@ -761,6 +762,11 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   uint32_t D, C, R;
   // FP64-2D streams carry a 4-byte-padded count section (encoder layout).
   section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
+  // A group whose sections do not fit the stream (corrupt input) is left
+  // undecoded rather than read out of bounds.
+  if (U == 0 || db > 8 * sizeof(Q) || cb > 16 || rb > 32 ||
+      P + (8ull + M) * a.G + off + D + C + R > a.stream_bytes)
+    return;
   const uint8_t* pc = pd + D;
   const uint8_t* pr = pd + D + C;
   const uint32_t B = (U + 7u) >> 3;
@@ -997,10 +1003,12 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
 }
 
 template <typename T, int DIM>
-int decompress_impl(const gpz_header* h, const void* dstream, void* dx,
-                    void* dy, void* dz, hipStream_t s) {
+int decompress_impl(const gpz_header* h, const void* dstream,
+                    uint64_t stream_bytes, void* dx, void* dy, void* dz,
+                    hipStream_t s) {
   DecodeArgs da;
   da.stream = static_cast<const uint8_t*>(dstream);
+  da.stream_bytes = stream_bytes;
   da.n = h->n;
   da.G = h->groups;
   da.eb = h->eb;
@@ -1131,11 +1139,11 @@ int gpz_decompress(const gpz_header* h, const void* d_stream,
     return GPZ_ERR_CORRUPT;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (h->float_bits == 32)
-    return h->dim == 3 ? decompress_impl<float, 3>(h, d_stream, d_x, d_y, d_z, s)
-                       : decompress_impl<float, 2>(h, d_stream, d_x, d_y, d_z, s);
+    return h->dim == 3 ? decompress_impl<float, 3>(h, d_stream, stream_bytes, d_x, d_y, d_z, s)
+                       : decompress_impl<float, 2>(h, d_stream, stream_bytes, d_x, d_y, d_z, s);
   if (h->float_bits == 64)
-    return h->dim == 3 ? decompress_impl<double, 3>(h, d_stream, d_x, d_y, d_z, s)
-                       : decompress_impl<double, 2>(h, d_stream, d_x, d_y, d_z, s);
+    return h->dim == 3 ? decompress_impl<double, 3>(h, d_stream, stream_bytes, d_x, d_y, d_z, s)
+                       : decompress_impl<double, 2>(h, d_stream, stream_bytes, d_x, d_y, d_z, s);
   return GPZ_ERR_CORRUPT;
 }
 
