@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPZ particle compressor on one MI355X.
+
+A step is one compress plus one decompress of the resident snapshot
+(143,735,721 clustered FP32 3D points, rel 1e-4 by default).  The timed
+region holds K steps bracketed by torch.cuda.synchronize(); inputs, stream,
+workspace and outputs stay in HBM and nothing is allocated inside it.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload NAME]
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("compress/decompress GB/s & % HBM roofline, 143.7M f32 3D pts; "
+          "batched vs per-call")
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def kernel_bytes(name, in_bytes, stream_bytes, payload_bytes, G, M, P):
+    """Algorithmic HBM bytes of one launch of each kernel."""
+    return {
+        "encode": in_bytes + payload_bytes + (M + 8) * G,
+        "decode": stream_bytes + in_bytes,
+        "compact": 2 * (payload_bytes + M * G) + 16 * G,
+        "scan": 16 * G + P,
+        "seg_sample": 1024 * 256 * (in_bytes // max(1, G * 1024)),
+        "seg_decide": 6 * 1024 * 8,
+    }.get(name, 0)
+
+
+def traffic_from_profiles(kernel, workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(f"{kernel}/{workload}")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(x, y, z, spec, eb, seg, threads, reps=2):
+    """The oracle (scalar/OpenMP C++ restatement) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import numpy as np
+    O.build()
+    used = O.num_threads(threads)
+    dt = x.dtype.type
+    nbytes = x.nbytes * (3 if z is not None else 2)
+
+    def once():
+        t0 = time.perf_counter()
+        s = O.compress(x, y, z, dt(eb), seg)
+        t1 = time.perf_counter()
+        O.decompress(s)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    once()
+    runs = sorted((once() for _ in range(reps)), key=lambda r: r[0] + r[1])
+    tc, td = runs[len(runs) // 2]
+    return dict(value=round(nbytes / (tc + td) / 1e9, 4), unit="GB/s",
+                cores=used, kind="port",
+                sample=(f"whole {spec['name']} snapshot ({x.size} points), "
+                        f"oracle compress+decompress, OpenMP {used} threads on "
+                        f"{cpu_model()}, median of {reps} after 1 warm-up"),
+                compress_GBps=round(nbytes / tc / 1e9, 4),
+                decompress_GBps=round(nbytes / td / 1e9, 4))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="large_f32_3d")
+    ap.add_argument("--n", type=int, default=None, help="override point count")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py runs on one GPU: this build has no multi-GPU path "
+                 "(the compressor shards by 1024-point groups; see DESIGN.md)")
+
+    import numpy as np
+    import torch
+    from gpz_amd import _native as N
+    from gpz_amd import api, datagen
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    x, y, z, spec = datagen.workload(args.workload, n=args.n)
+    spec["name"] = args.workload
+    log(f"[bench] generated {args.workload}: {x.size} points in "
+        f"{time.time() - t0:.1f}s")
+    dim, bits, n = spec["dim"], spec["bits"], x.size
+    in_bytes = x.nbytes * dim
+    planes = [torch.from_numpy(a).to(dev) for a in (x, y, z) if a is not None]
+    dx, dy = planes[0], planes[1]
+    dz = planes[2] if dim == 3 else None
+    torch.cuda.synchronize()
+
+    eb = api.resolve_eb(dx, dy, dz, spec["mode"], spec["e"])
+    rel = float(np.float32(spec["e"])) if bits == 32 else float(spec["e"])
+    p = api.make_params(dim, bits, n, eb, rel, spec["mode"], spec["seg"])
+    out = torch.empty(api.stream_bound(p), dtype=torch.uint8, device=dev)
+    ws = torch.empty(api.workspace_bytes(p), dtype=torch.uint8, device=dev)
+    dt = torch.float64 if bits == 64 else torch.float32
+    ox = torch.empty(n, dtype=dt, device=dev)
+    oy = torch.empty(n, dtype=dt, device=dev)
+    oz = torch.empty(n, dtype=dt, device=dev) if dim == 3 else None
+
+    api.compress_into(p, dx, dy, dz, out, ws)
+    res = api.compress_result(ws)
+    N.check(res.status, "compress")
+    nbytes = int(res.stream_bytes)
+    hdr = api.read_header(out)
+    log(f"[bench] eb={eb:.9g} seg={res.seg_size} stream={nbytes} B "
+        f"ratio={in_bytes / nbytes:.3f}")
+
+    def step(ev=None):
+        if ev:
+            ev[0].record()
+        api.compress_into(p, dx, dy, dz, out, ws)
+        if ev:
+            ev[1].record()
+        api.decompress_into(hdr, out, nbytes, ox, oy, oz)
+        if ev:
+            ev[2].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done")
+
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)]
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    tc = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+    td = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+    ms_step = elapsed * 1e3 / args.steps
+    log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step "
+        f"(compress {tc:.3f} ms, decompress {td:.3f} ms)")
+
+    # Per-kernel durations with hipEvents on the launch stream.
+    L = N.lib()
+    L.gpz_timing_reset()
+    L.gpz_timing_enable(1)
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    L.gpz_timing_enable(0)
+    G = (n + 1023) // 1024
+    P, M = (32, 64) if bits == 64 else (24, 36)
+    payload = nbytes - P - (8 + M) * G
+    kernels = {}
+    import ctypes
+    for name in ("seg_sample", "seg_decide", "encode", "scan", "compact",
+                 "decode"):
+        ms = ctypes.c_double(0)
+        cnt = ctypes.c_uint64(0)
+        L.gpz_timing_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value:
+            avg = ms.value / cnt.value
+            b = kernel_bytes(name, in_bytes, nbytes, payload, G, M, P)
+            kernels[name] = dict(avg_ms=round(avg, 4),
+                                 bytes=int(b),
+                                 GBps=round(b / (avg * 1e-3) / 1e9, 1))
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    kd = kernels[dom]
+    roof = dict(bound="hbm", achieved=kd["GBps"], peak=HBM_PEAK_GBPS,
+                unit="GB/s", frac=round(kd["GBps"] / HBM_PEAK_GBPS, 4),
+                traffic=traffic_from_profiles(dom, args.workload),
+                kernel=dom, avg_ms=kd["avg_ms"], algorithmic_bytes=kd["bytes"])
+    log(f"[bench] kernels: {json.dumps(kernels)}")
+
+    # correctness spot check of the timed output (decode of the last step)
+    assert torch.isfinite(ox).all().item()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(x, y, z, spec, eb, res.seg_size, args.cpu_threads)
+        log(f"[bench] cpu baseline {cpu}")
+
+    value = in_bytes / (ms_step * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if bits == 64 else "f32",
+        "data": ("synthetic: deterministic clustered generator "
+                 f"({spec['clusters']} Gaussian clusters, {int(spec['frac'] * 100)}% "
+                 f"clustered, box {spec['box']:g}, seed {spec['seed']})"),
+        "config": {"workload": args.workload, "n_points": n, "dim": dim,
+                   "float_bits": bits, "mode": spec["mode"], "e": spec["e"],
+                   "eb_abs": eb, "seg_size": int(res.seg_size),
+                   "step": "compress + decompress",
+                   "value_definition": "uncompressed bytes / step time"},
+        "compress_GBps": round(in_bytes / (tc * 1e-3) / 1e9, 3),
+        "decompress_GBps": round(in_bytes / (td * 1e-3) / 1e9, 3),
+        "compress_ms": round(tc, 4),
+        "decompress_ms": round(td, 4),
+        "ratio": round(in_bytes / nbytes, 4),
+        "stream_bytes": nbytes,
+        "roofline": roof,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -36,7 +36,8 @@ EXPORTS = (
     "gpz_version", "gpz_stream_bound", "gpz_compress_workspace_bytes",
     "gpz_compress", "gpz_compress_result", "gpz_parse_header",
     "gpz_read_header", "gpz_decompress", "gpz_min_max_workspace_bytes",
-    "gpz_min_max", "gpz_rel_to_abs",
+    "gpz_min_max", "gpz_rel_to_abs", "gpz_timing_enable", "gpz_timing_reset",
+    "gpz_timing_read",
 )
 
 
@@ -87,6 +88,9 @@ def _bind(L):
         "gpz_min_max_workspace_bytes": ([], u64),
         "gpz_min_max": ([u32, u32, u64, vp, vp, vp, vp, vp, vp], i32),
         "gpz_rel_to_abs": ([u32, ctypes.c_double, vp], ctypes.c_double),
+        "gpz_timing_enable": ([i32], i32),
+        "gpz_timing_reset": ([], i32),
+        "gpz_timing_read": ([ctypes.c_char_p, P(ctypes.c_double), P(u64)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -31,9 +31,46 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "gpz.h"
 
 namespace {
+
+// ------------------------------------------------ optional kernel timing
+// When enabled (gpz_timing_enable), every launch is bracketed by a pair of
+// hipEvents on its own stream; gpz_timing_read sums the elapsed times.
+struct TimingState {
+  std::mutex mu;
+  bool on = false;
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::map<std::string, std::pair<double, uint64_t>> totals;
+};
+TimingState& timing() {
+  static TimingState t;
+  return t;
+}
+struct LaunchTimer {
+  const char* name;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  LaunchTimer(const char* n, hipStream_t st) : name(n), s(st) {
+    if (!timing().on) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
+        hipEventRecord(a, s) != hipSuccess)
+      a = b = nullptr;
+  }
+  ~LaunchTimer() {
+    if (!a) return;
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> g(timing().mu);
+    timing().pending.push_back({name, a, b});
+  }
+};
 
 constexpr uint32_t kGroup = 1024;
 constexpr uint32_t kSlot = 9344;       // max payload bytes of one group
@@ -959,6 +996,7 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
     const uint64_t stride = n - kSampleLen;  // u64 wrap for n < 256, as the reference
     uint64_t st = stride / kSampleBlocks;
     if (st < 1) st = 1;
+    LaunchTimer lt("seg_sample", s);
     hipLaunchKernelGGL((seg_sample_kernel<T, DIM>), dim3(kSampleBlocks), dim3(256), 0, s,
                        static_cast<const T*>(dx), static_cast<const T*>(dy),
                        static_cast<const T*>(dz), n, st, win);
@@ -971,7 +1009,10 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   sa.rel_small = rel <= 1e-4;
   sa.sampled = sample;
   sa.eb = sizeof(T) == 4 ? static_cast<double>(static_cast<float>(p->eb)) : p->eb;
-  hipLaunchKernelGGL((seg_decide_kernel<T, DIM>), dim3(1), dim3(256), 0, s, win, sa, info);
+  {
+    LaunchTimer lt("seg_decide", s);
+    hipLaunchKernelGGL((seg_decide_kernel<T, DIM>), dim3(1), dim3(256), 0, s, win, sa, info);
+  }
 
   EncodeArgs ea;
   ea.x = dx; ea.y = dy; ea.z = dz;
@@ -980,7 +1021,10 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   ea.meta = ws + L.meta;
   ea.loc = reinterpret_cast<uint64_t*>(ws + L.loc);
   ea.slots = ws + L.slots;
-  hipLaunchKernelGGL((encode_kernel<T, DIM>), dim3(static_cast<uint32_t>(G)), dim3(64), 0, s, ea);
+  {
+    LaunchTimer lt("encode", s);
+    hipLaunchKernelGGL((encode_kernel<T, DIM>), dim3(static_cast<uint32_t>(G)), dim3(64), 0, s, ea);
+  }
 
   ScanArgs sc;
   sc.loc = ea.loc;
@@ -988,7 +1032,10 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   sc.stream = static_cast<uint8_t*>(dstream);
   sc.info = info;
   sc.n = n; sc.G = G; sc.dim = DIM; sc.eb = sa.eb;
-  hipLaunchKernelGGL((scan_kernel<T>), dim3(1), dim3(1024), 0, s, sc);
+  {
+    LaunchTimer lt("scan", s);
+    hipLaunchKernelGGL((scan_kernel<T>), dim3(1), dim3(1024), 0, s, sc);
+  }
 
   CompactArgs ca;
   ca.loc = ea.loc;
@@ -998,7 +1045,10 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   ca.stream = sc.stream;
   ca.info = info;
   ca.G = G;
-  hipLaunchKernelGGL((compact_kernel<T>), dim3(static_cast<uint32_t>(G)), dim3(256), 0, s, ca);
+  {
+    LaunchTimer lt("compact", s);
+    hipLaunchKernelGGL((compact_kernel<T>), dim3(static_cast<uint32_t>(G)), dim3(256), 0, s, ca);
+  }
   return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
 }
 
@@ -1014,8 +1064,11 @@ int decompress_impl(const gpz_header* h, const void* dstream,
   da.eb = h->eb;
   da.seg = h->seg_size;
   da.x = dx; da.y = dy; da.z = dz;
-  hipLaunchKernelGGL((decode_kernel<T, DIM>), dim3(static_cast<uint32_t>(h->groups)),
-                     dim3(64), 0, s, da);
+  {
+    LaunchTimer lt("decode", s);
+    hipLaunchKernelGGL((decode_kernel<T, DIM>), dim3(static_cast<uint32_t>(h->groups)),
+                       dim3(64), 0, s, da);
+  }
   return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
 }
 
@@ -1163,6 +1216,45 @@ int gpz_min_max(uint32_t dim, uint32_t float_bits, uint64_t n,
     return dim == 3 ? minmax_impl<double, 3>(n, d_x, d_y, d_z, d_ws, mm, s)
                     : minmax_impl<double, 2>(n, d_x, d_y, d_z, d_ws, mm, s);
   return GPZ_ERR_INVALID_ARG;
+}
+
+int gpz_timing_enable(int on) {
+  std::lock_guard<std::mutex> g(timing().mu);
+  timing().on = on != 0;
+  return GPZ_OK;
+}
+
+int gpz_timing_reset(void) {
+  std::lock_guard<std::mutex> g(timing().mu);
+  for (auto& r : timing().pending) {
+    (void)hipEventSynchronize(r.b);
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  timing().pending.clear();
+  timing().totals.clear();
+  return GPZ_OK;
+}
+
+int gpz_timing_read(const char* kernel, double* total_ms, uint64_t* launches) {
+  if (!kernel || !total_ms || !launches) return GPZ_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> g(timing().mu);
+  for (auto& r : timing().pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) != hipSuccess ||
+        hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess)
+      return GPZ_ERR_HIP;
+    auto& t = timing().totals[r.name];
+    t.first += ms;
+    t.second += 1;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  timing().pending.clear();
+  auto it = timing().totals.find(kernel);
+  *total_ms = it == timing().totals.end() ? 0.0 : it->second.first;
+  *launches = it == timing().totals.end() ? 0 : it->second.second;
+  return GPZ_OK;
 }
 
 double gpz_rel_to_abs(uint32_t float_bits, double e, const void* mm) {

@@ -133,6 +133,16 @@ int gpz_min_max(uint32_t dim, uint32_t float_bits, uint64_t n,
  * in float for FP32 and in double for FP64). */
 double gpz_rel_to_abs(uint32_t float_bits, double e, const void* mm);
 
+/* Per-kernel timing for benchmarks: while enabled, each kernel launch is
+ * bracketed by hipEvents on its stream.  gpz_timing_read returns the summed
+ * elapsed milliseconds and launch count of one kernel ("seg_sample",
+ * "seg_decide", "encode", "scan", "compact", "decode"), waiting for pending
+ * events.  The reference marks the same regions with NVTX ranges
+ * (src/cudaWarpCompress.cu:1577,1626,1747,1817). */
+int gpz_timing_enable(int on);
+int gpz_timing_reset(void);
+int gpz_timing_read(const char* kernel, double* total_ms, uint64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif
