@@ -176,6 +176,60 @@ __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, uint32_t lane) {
   return v;
 }
 
+// Bit-matrix transpose of 16 words, two 16x16 blocks at once: on entry
+// a[i] holds item i (bits 0..31); on exit a[b] bits 0..15 hold bit b of
+// items 0..15 and bits 16..31 hold bit b+16 of items 0..15.  The operation
+// is its own inverse.  4 stages x 8 swaps of block-masked xor.
+__device__ __forceinline__ void transpose16x2(uint32_t (&a)[16]) {
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const int j = 8 >> st;
+    const uint32_t m = st == 0 ? 0x00FF00FFu : st == 1 ? 0x0F0F0F0Fu
+                     : st == 2 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k & j) continue;
+      const uint32_t t = ((a[k] >> j) ^ a[k + j]) & m;
+      a[k + j] ^= t;
+      a[k] ^= t << j;
+    }
+  }
+}
+// Word of bit plane b (0..31) after transpose16x2.
+__device__ __forceinline__ uint32_t plane_word(const uint32_t (&a)[16], int b) {
+  return b < 16 ? (a[b] & 0xFFFFu) : (a[b - 16] >> 16);
+}
+
+// Exact unsigned 32-bit division by a per-group divisor d >= 1
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", 1994, fig. 4.1): q = (t + ((n - t) >> s1)) >> s2 with
+// t = mulhi(m, n).
+struct Div32 {
+  uint32_t m, s1, s2;
+};
+__device__ __forceinline__ Div32 div32_make(uint32_t d) {
+  const uint32_t l = d > 1 ? 32u - __builtin_clz(d - 1u) : 0u;  // ceil(log2 d)
+  const uint64_t num = (static_cast<uint64_t>(1) << 32) *
+                       ((static_cast<uint64_t>(1) << l) - d);
+  Div32 r;
+  r.m = static_cast<uint32_t>(num / d + 1u);
+  r.s1 = l < 1 ? l : 1u;
+  r.s2 = l > 0 ? l - 1u : 0u;
+  return r;
+}
+__device__ __forceinline__ uint32_t div32(uint32_t n, Div32 v) {
+  const uint32_t t = __umulhi(v.m, n);
+  return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+// Quotient for the decoder: divisor 0 follows udiv; wide operands (FP64
+// streams only) take the slow path.
+template <typename Q>
+__device__ __forceinline__ Q qdiv(Q n, Q d, Div32 v) {
+  if (d == 0) return ~Q(0);
+  if (sizeof(Q) == 8 && ((n >> 32) != 0 || (d >> 32) != 0)) return n / d;
+  return static_cast<Q>(div32(static_cast<uint32_t>(n), v));
+}
+
 // Section sizes (src/cudaWarpCompress.cu:324-329; FP64 3D
 // src/cudaWarpCompressDouble.cu:325-329, FP64 2D :686-692).
 template <typename T, int DIM>
@@ -215,7 +269,7 @@ __device__ __forceinline__ void block_counts(T mn0, T mx0, T mn1, T mx1,
 
 // ------------------------------------------------------- workspace layout
 struct WsLayout {
-  uint64_t info, win, loc, off, meta, slots, total;
+  uint64_t info, win, loc, off, bsum, meta, slots, total;
 };
 __host__ __device__ inline uint64_t align256(uint64_t v) {
   return (v + 255) & ~255ull;
@@ -227,7 +281,8 @@ inline WsLayout ws_layout(uint32_t bits, uint64_t G) {
   w.win = 256;
   w.loc = align256(w.win + 6ull * kSampleBlocks * 8);
   w.off = align256(w.loc + 8 * G);
-  w.meta = align256(w.off + 8 * G);
+  w.bsum = align256(w.off + 8 * G);
+  w.meta = align256(w.bsum + 8 * ((G + 1023) / 1024));
   w.slots = align256(w.meta + M * G);
   w.total = align256(w.slots + static_cast<uint64_t>(kSlot) * G);
   return w;
@@ -607,35 +662,48 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
   for (uint32_t o = 16u * lane; o < loc16; o += 1024u)
     *reinterpret_cast<uint4*>(stg + o) = make_uint4(0, 0, 0, 0);
   __syncthreads();
-  for (uint32_t p = 0; p < db; ++p) {
-    const uint32_t b = db - 1u - p;
-    uint32_t b0 = 0, b1 = 0;
+  // Delta and count planes: transpose the lane's 16 runs so that each
+  // 16-bit plane word holds runs 16l..16l+15 (bit t = run 16l+t); bfrev
+  // turns it into the two MSB-first bytes 2l, 2l+1 of the plane.
+  {
+    uint32_t t16[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      b0 |= ((del[u] >> b) & 1u) << (7 - u);
-      b1 |= ((del[8 + u] >> b) & 1u) << (7 - u);
+    for (int t = 0; t < 16; ++t) t16[t] = del[t];
+    transpose16x2(t16);
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      if (b < static_cast<int>(db)) {
+        const uint32_t bf = __brev(plane_word(t16, b));
+        uint8_t* dst = stg + (db - 1u - b) * B + 2u * lane;
+        if (2u * lane < B) dst[0] = static_cast<uint8_t>(bf >> 24);
+        if (2u * lane + 1u < B) dst[1] = static_cast<uint8_t>(bf >> 16);
+      }
     }
-    if (2u * lane < B) stg[p * B + 2u * lane] = static_cast<uint8_t>(b0);
-    if (2u * lane + 1u < B) stg[p * B + 2u * lane + 1u] = static_cast<uint8_t>(b1);
-  }
-  for (uint32_t p = 0; p < cb; ++p) {
-    const uint32_t b = cb - 1u - p;
-    uint32_t b0 = 0, b1 = 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      b0 |= ((cnt[u] >> b) & 1u) << (7 - u);
-      b1 |= ((cnt[8 + u] >> b) & 1u) << (7 - u);
+    for (int t = 0; t < 16; ++t) t16[t] = cnt[t];
+    transpose16x2(t16);
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      if (b < static_cast<int>(cb)) {
+        const uint32_t bf = __brev(plane_word(t16, b));
+        uint8_t* dst = stg + D + (cb - 1u - b) * B + 2u * lane;
+        if (2u * lane < B) dst[0] = static_cast<uint8_t>(bf >> 24);
+        if (2u * lane + 1u < B) dst[1] = static_cast<uint8_t>(bf >> 16);
+      }
     }
-    if (2u * lane < B) stg[D + p * B + 2u * lane] = static_cast<uint8_t>(b0);
-    if (2u * lane + 1u < B) stg[D + p * B + 2u * lane + 1u] = static_cast<uint8_t>(b1);
   }
-  for (uint32_t p = 0; p < rb; ++p) {
-    const uint32_t b = rb - 1u - p;
-    uint32_t wv = 0;
+  // Rel planes: LSB-first over rank, so the plane word is stored as is.
+  {
+    uint32_t t16[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) wv |= ((val[k] >> b) & 1u) << k;
-    *reinterpret_cast<uint16_t*>(stg + D + C + p * 128u + 2u * lane) =
-        static_cast<uint16_t>(wv);
+    for (int k = 0; k < 16; ++k) t16[k] = val[k];
+    transpose16x2(t16);
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      if (b < static_cast<int>(rb))
+        *reinterpret_cast<uint16_t*>(stg + D + C + (rb - 1u - b) * 128u + 2u * lane) =
+            static_cast<uint16_t>(plane_word(t16, b));
+    }
   }
   __syncthreads();
 
@@ -645,8 +713,9 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
     uint32_t word = 0;
     if (sizeof(T) == 4) {
       if (lane < 6) {
-        const float f = static_cast<float>(lane < 3 ? mn[lane] : mx[lane - 3]);
-        word = __float_as_uint(f);
+        const T v = lane == 0 ? mn[0] : lane == 1 ? mn[1] : lane == 2 ? mn[2]
+                  : lane == 3 ? mx[0] : lane == 4 ? mx[1] : mx[2];
+        word = __float_as_uint(static_cast<float>(v));
       } else if (lane == 6) {
         word = outlier;
       } else if (lane == 7) {
@@ -657,7 +726,9 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
     } else {
       if (lane < 12) {
         const uint32_t e = lane >> 1;
-        const double d = static_cast<double>(e < 3 ? mn[e] : mx[e - 3]);
+        const T v = e == 0 ? mn[0] : e == 1 ? mn[1] : e == 2 ? mn[2]
+                  : e == 3 ? mx[0] : e == 4 ? mx[1] : mx[2];
+        const double d = static_cast<double>(v);
         const uint64_t u = static_cast<uint64_t>(__double_as_longlong(d));
         word = (lane & 1) ? static_cast<uint32_t>(u >> 32) : static_cast<uint32_t>(u);
       } else if (lane == 12) {
@@ -680,9 +751,16 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
 }
 
 // ------------------------------------------------- offsets scan + header
+// Exclusive scan of the G payload sizes (the reference's CUB DeviceScan,
+// src/cudaWarpCompress.cu:1658-1666) in three steps over blocks of 1024
+// groups: block sums, a scan of the block sums (which also writes the
+// header and the total size), and a down-sweep writing every offset.
+constexpr uint32_t kScanBlock = 1024;
+
 struct ScanArgs {
   const uint64_t* loc;
   uint64_t* off;
+  uint64_t* bsum;
   uint8_t* stream;
   DevInfo* info;
   uint64_t n, G;
@@ -690,35 +768,53 @@ struct ScanArgs {
   double eb;
 };
 
-template <typename T>
-__global__ __launch_bounds__(1024) void scan_kernel(ScanArgs a) {
-  __shared__ uint64_t wsum[16];
+// Block-wide (256 threads) exclusive scan; *total receives the block sum.
+__device__ __forceinline__ uint64_t block_excl_scan256(uint64_t v,
+                                                      uint64_t* total,
+                                                      uint64_t* sh) {
   const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const uint64_t incl = wave_incl_scan(v, lane);
+  if (lane == 63) sh[w] = incl;
+  __syncthreads();
+  uint64_t base = 0, tot = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    base += i < w ? sh[i] : 0;
+    tot += sh[i];
+  }
+  *total = tot;
+  return base + incl - v;
+}
+
+__global__ __launch_bounds__(256) void scan_reduce_kernel(ScanArgs a) {
+  __shared__ uint64_t sh[4];
+  if (a.info->status != GPZ_OK) return;
+  const uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * kScanBlock + 4 * threadIdx.x;
+  uint64_t v = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v += i0 + i < a.G ? a.loc[i0 + i] : 0;
+  uint64_t tot;
+  block_excl_scan256(v, &tot, sh);
+  if (threadIdx.x == 0) a.bsum[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scan_top_kernel(ScanArgs a, uint32_t nb) {
+  __shared__ uint64_t sh[4];
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
   if (a.info->status != GPZ_OK) return;
-  const uint64_t per = (a.G + 1023) / 1024;
-  const uint64_t b0 = t * per;
-  const uint64_t b1 = b0 + per < a.G ? b0 + per : a.G;
-  uint64_t s = 0;
-  for (uint64_t i = b0; i < b1; ++i) s += a.loc[i];
-  const uint64_t incl = wave_incl_scan(s, lane);
-  if (lane == 63) wsum[w] = incl;
-  __syncthreads();
-  uint64_t wbase = 0;
-  for (uint32_t i = 0; i < w; ++i) wbase += wsum[i];
-  uint64_t run = wbase + incl - s;
-  uint64_t* soff = reinterpret_cast<uint64_t*>(a.stream + P);
-  for (uint64_t i = b0; i < b1; ++i) {
-    a.off[i] = run;
-    soff[i] = run;
-    run += a.loc[i];
+  uint64_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += 256) {
+    const uint32_t i = base + threadIdx.x;
+    const uint64_t v = i < nb ? a.bsum[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan256(v, &tot, sh);
+    if (i < nb) a.bsum[i] = carry + ex;
+    carry += tot;
+    __syncthreads();
   }
-  if (t == 1023) {
-    uint64_t total = 0;
-    for (int i = 0; i < 16; ++i) total += wsum[i];
-    a.info->stream_bytes = P + (8ull + M) * a.G + total;
-  }
-  if (t == 0) {
+  if (threadIdx.x == 0) {
+    a.info->stream_bytes = P + (8ull + M) * a.G + carry;
     uint32_t* h = reinterpret_cast<uint32_t*>(a.stream);
     h[0] = a.dim;
     h[1] = static_cast<uint32_t>(a.n);
@@ -734,6 +830,31 @@ __global__ __launch_bounds__(1024) void scan_kernel(ScanArgs a) {
       h[6] = a.info->seg_size;
       h[7] = 32;   // parameter_size, src/cudaWarpCompressDouble.cu:1674-1680
     }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scan_down_kernel(ScanArgs a) {
+  __shared__ uint64_t sh[4];
+  constexpr uint32_t P = Prec<T>::kHeader;
+  if (a.info->status != GPZ_OK) return;
+  const uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * kScanBlock + 4 * threadIdx.x;
+  uint64_t v[4], s = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[i] = i0 + i < a.G ? a.loc[i0 + i] : 0;
+    s += v[i];
+  }
+  uint64_t tot;
+  uint64_t run = a.bsum[blockIdx.x] + block_excl_scan256(s, &tot, sh);
+  uint64_t* soff = reinterpret_cast<uint64_t*>(a.stream + P);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    if (i0 + i < a.G) {
+      a.off[i0 + i] = run;
+      soff[i0 + i] = run;
+    }
+    run += v[i];
   }
 }
 
@@ -776,10 +897,16 @@ struct DecodeArgs {
   void* z;
 };
 
+// One wave per group: the group's payload is staged in LDS with coalesced
+// dword loads, the planes are turned back into per-run deltas / counts and
+// per-rank relative positions with the same 16x16 bit transposes the
+// encoder uses, block ids are split with per-group magic divisors, and each
+// lane writes 16 consecutive output points.
 template <typename T, int DIM>
 __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   using Q = typename Prec<T>::Q;
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  __shared__ __align__(16) uint32_t s_pay[kSlot / 4];
   __shared__ __align__(16) Q s_bid[kGroup];
   __shared__ __align__(16) uint8_t s_start[kGroup];
 
@@ -791,22 +918,42 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   for (int i = 0; i < 6; ++i) mm[i] = reinterpret_cast<const T*>(meta)[i];
   const Q outlier = *reinterpret_cast<const Q*>(meta + 6 * sizeof(T));
   const uint16_t* u16 = reinterpret_cast<const uint16_t*>(meta + 6 * sizeof(T) + sizeof(Q));
-  const uint32_t db = u16[0], cb = u16[1], rb = u16[2];
-  uint32_t U = u16[3];
-  if (U > kGroup) U = kGroup;
+  const uint32_t db = u16[0], cb = u16[1], rb = u16[2], U = u16[3];
   const uint64_t off = reinterpret_cast<const uint64_t*>(a.stream + P)[c];
-  const uint8_t* pd = a.stream + P + (8ull + M) * a.G + off;
   uint32_t D, C, R;
   // FP64-2D streams carry a 4-byte-padded count section (encoder layout).
   section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
-  // A group whose sections do not fit the stream (corrupt input) is left
-  // undecoded rather than read out of bounds.
-  if (U == 0 || db > 8 * sizeof(Q) || cb > 16 || rb > 32 ||
-      P + (8ull + M) * a.G + off + D + C + R > a.stream_bytes)
+  // A group whose sections do not fit the stream or the format's limits
+  // (corrupt input) is left undecoded rather than read out of bounds.
+  const uint64_t pay0 = P + (8ull + M) * a.G + off;
+  if (U == 0 || U > kGroup || db > 32 || cb > 16 || rb > 32 ||
+      pay0 + D + C + R > a.stream_bytes)
     return;
-  const uint8_t* pc = pd + D;
-  const uint8_t* pr = pd + D + C;
+  const uint32_t loc = D + C + R;
+
+  // ---- stage the payload (4-byte aligned) into LDS
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.stream + pay0);
+    const uint32_t nw = loc >> 2;
+    for (uint32_t base = 0; base < nw; base += 512) {
+      uint32_t v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t i = base + 64u * k + lane;
+        v[k] = i < nw ? src[i] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t i = base + 64u * k + lane;
+        if (i < nw) s_pay[i] = v[k];
+      }
+    }
+  }
+  reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  const uint8_t* pb = reinterpret_cast<const uint8_t*>(s_pay);
   const uint32_t B = (U + 7u) >> 3;
+
   const T eb = static_cast<T>(a.eb);
   const uint32_t seg = a.seg;
   const T seg2eb = static_cast<T>(2u * seg) * eb;
@@ -816,30 +963,39 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   const uint32_t nvalid =
       static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
 
-  // runs 16l..16l+15: deltas and counts from MSB-first planes
-  Q del[16];
-  uint32_t cnt[16];
+  // ---- runs 16l..16l+15: deltas and counts from the MSB-first planes
+  uint32_t del[16], cnt[16];
+  {
+    uint32_t t16[16];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) { del[t] = 0; cnt[t] = 0; }
-  for (uint32_t p = 0; p < db; ++p) {
-    const uint32_t b = db - 1u - p;
-    const uint32_t b0 = 2u * lane < B ? pd[p * B + 2u * lane] : 0u;
-    const uint32_t b1 = 2u * lane + 1u < B ? pd[p * B + 2u * lane + 1u] : 0u;
+    for (int t = 0; t < 16; ++t) t16[t] = 0;
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      del[u] |= static_cast<Q>((b0 >> (7 - u)) & 1u) << b;
-      del[8 + u] |= static_cast<Q>((b1 >> (7 - u)) & 1u) << b;
+    for (int b = 0; b < 32; ++b) {
+      if (b < static_cast<int>(db)) {
+        const uint8_t* q = pb + (db - 1u - b) * B + 2u * lane;
+        const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
+        const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
+        const uint32_t w = __brev((b0 << 24) | (b1 << 16));
+        t16[b & 15] |= b < 16 ? w : (w << 16);
+      }
     }
-  }
-  for (uint32_t p = 0; p < cb; ++p) {
-    const uint32_t b = cb - 1u - p;
-    const uint32_t b0 = 2u * lane < B ? pc[p * B + 2u * lane] : 0u;
-    const uint32_t b1 = 2u * lane + 1u < B ? pc[p * B + 2u * lane + 1u] : 0u;
+    transpose16x2(t16);
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      cnt[u] |= ((b0 >> (7 - u)) & 1u) << b;
-      cnt[8 + u] |= ((b1 >> (7 - u)) & 1u) << b;
+    for (int t = 0; t < 16; ++t) del[t] = t16[t];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) t16[t] = 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      if (b < static_cast<int>(cb)) {
+        const uint8_t* q = pb + D + (cb - 1u - b) * B + 2u * lane;
+        const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
+        const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
+        t16[b] = __brev((b0 << 24) | (b1 << 16));
+      }
     }
+    transpose16x2(t16);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) cnt[t] = t16[t];
   }
   // block id per run = outlier + prefix of deltas (delta of run 0 ignored,
   // src/cudaWarpCompress.cu:930-935); run end = prefix of counts.
@@ -850,19 +1006,17 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const uint32_t i = 16u * lane + t;
-    const Q d = (i == 0) ? outlier : (i < U ? del[t] : Q(0));
+    const Q d = (i == 0) ? outlier : (i < U ? static_cast<Q>(del[t]) : Q(0));
     const uint32_t cn = i < U ? cnt[t] : 0u;
     bsum += d;
     csum += cn;
     bid[t] = bsum;
     cend[t] = csum;
   }
-  const Q bincl = wave_incl_scan(static_cast<uint64_t>(bsum), lane);
+  const Q bincl = static_cast<Q>(wave_incl_scan(static_cast<uint64_t>(bsum), lane));
   const uint32_t cincl = wave_incl_scan(csum, lane);
   const Q bpre = static_cast<Q>(bincl - bsum);
   const uint32_t cpre = cincl - csum;
-  reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
-  __syncthreads();
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const uint32_t i = 16u * lane + t;
@@ -872,8 +1026,24 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
       if (st < kGroup) s_start[st] = 1;
     }
   }
+  // ---- relative positions of ranks 16l..16l+15 (LSB-first planes)
+  uint32_t rel[16];
+  {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) rel[k] = 0;
+    const uint8_t* pr = pb + D + C;
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      if (b < static_cast<int>(rb)) {
+        const uint32_t w =
+            *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
+        rel[b & 15] |= b < 16 ? w : (w << 16);
+      }
+    }
+    transpose16x2(rel);
+  }
   __syncthreads();
-  // rank -> run index by a prefix count of run starts
+  // ---- rank -> run index by a prefix count of run starts
   const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
   const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
   uint32_t rs = 0;
@@ -883,43 +1053,62 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
     rs += (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
     runof[k] = rs;
   }
-  const uint32_t rincl = wave_incl_scan(rs, lane);
-  const uint32_t rpre = rincl - rs;
-  // relative positions from LSB-first planes
-  Q rel[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) rel[k] = 0;
-  for (uint32_t p = 0; p < rb; ++p) {
-    const uint32_t b = rb - 1u - p;
-    const uint32_t wv = *reinterpret_cast<const uint16_t*>(pr + p * 128u + 2u * lane);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) rel[k] |= static_cast<Q>((wv >> k) & 1u) << b;
-  }
+  const uint32_t rpre = wave_incl_scan(rs, lane) - rs;
+
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
   const Q m = static_cast<Q>(seg - 1u);
   const Q sq = static_cast<Q>(seg);
   const Q nxy = static_cast<Q>(nbx * nby);
-  T* X = static_cast<T*>(a.x);
-  T* Y = static_cast<T*>(a.y);
-  T* Z = static_cast<T*>(a.z);
+  const Div32 dvx = div32_make(static_cast<uint32_t>(nbx ? nbx : 1));
+  const Div32 dvy = div32_make(static_cast<uint32_t>(nby ? nby : 1));
+  const Div32 dvxy = div32_make(static_cast<uint32_t>(nxy ? nxy : 1));
+  T ox[16], oy[16], oz[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const uint32_t r = 16u * lane + k;
-    if (r < nvalid) {
-      uint32_t run = runof[k] + rpre;
-      run = run ? run - 1u : 0u;
-      if (run >= U) run = U ? U - 1u : 0u;
-      const Q b = s_bid[run];
-      const Q bx = urem<Q>(b, nbx);
-      const Q by = urem<Q>(udiv<Q>(b, nbx), nby);
-      const Q qx = bx * sq + (rel[k] & m);
-      const Q qy = by * sq + (shr(rel[k], sh) & m);
-      X[base + r] = dfma(static_cast<T>((qx << 1) | 1u), eb, mm[0]);
-      Y[base + r] = dfma(static_cast<T>((qy << 1) | 1u), eb, mm[1]);
-      if (DIM == 3) {
-        const Q bz = udiv<Q>(b, nxy);
-        const Q qz = bz * sq + shr(rel[k], 2u * sh);
-        Z[base + r] = dfma(static_cast<T>((qz << 1) | 1u), eb, mm[2]);
+    uint32_t run = runof[k] + rpre;
+    run = run ? run - 1u : 0u;
+    if (run >= U) run = U - 1u;
+    const Q b = s_bid[run];
+    const Q qb = qdiv<Q>(b, nbx, dvx);            // b / nbx
+    const Q bx = nbx ? b - qb * nbx : b;          // b % nbx
+    const Q by = nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb;
+    const Q rl = static_cast<Q>(rel[k]);
+    const Q qx = bx * sq + (rl & m);
+    const Q qy = by * sq + (shr(rl, sh) & m);
+    ox[k] = dfma(static_cast<T>((qx << 1) | 1u), eb, mm[0]);
+    oy[k] = dfma(static_cast<T>((qy << 1) | 1u), eb, mm[1]);
+    if (DIM == 3) {
+      const Q bz = qdiv<Q>(b, nxy, dvxy);
+      const Q qz = bz * sq + shr(rl, 2u * sh);
+      oz[k] = dfma(static_cast<T>((qz << 1) | 1u), eb, mm[2]);
+    }
+  }
+  // ---- store ranks 16l..16l+15 of each plane
+  T* X = static_cast<T*>(a.x) + base + 16u * lane;
+  T* Y = static_cast<T*>(a.y) + base + 16u * lane;
+  T* Z = static_cast<T*>(a.z) + base + 16u * lane;
+  if (16u * lane + 16u <= nvalid) {
+    constexpr int V = 16 / sizeof(T);  // elements per 16-byte store
+#pragma unroll
+    for (int k = 0; k < 16; k += V) {
+      if (sizeof(T) == 4) {
+        *reinterpret_cast<float4*>(X + k) = make_float4(ox[k], ox[k + 1], ox[k + 2], ox[k + 3]);
+        *reinterpret_cast<float4*>(Y + k) = make_float4(oy[k], oy[k + 1], oy[k + 2], oy[k + 3]);
+        if (DIM == 3)
+          *reinterpret_cast<float4*>(Z + k) = make_float4(oz[k], oz[k + 1], oz[k + 2], oz[k + 3]);
+      } else {
+        *reinterpret_cast<double2*>(X + k) = make_double2(ox[k], ox[k + 1]);
+        *reinterpret_cast<double2*>(Y + k) = make_double2(oy[k], oy[k + 1]);
+        if (DIM == 3) *reinterpret_cast<double2*>(Z + k) = make_double2(oz[k], oz[k + 1]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (16u * lane + k < nvalid) {
+        X[k] = ox[k];
+        Y[k] = oy[k];
+        if (DIM == 3) Z[k] = oz[k];
       }
     }
   }
@@ -1029,12 +1218,16 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   ScanArgs sc;
   sc.loc = ea.loc;
   sc.off = reinterpret_cast<uint64_t*>(ws + L.off);
+  sc.bsum = reinterpret_cast<uint64_t*>(ws + L.bsum);
   sc.stream = static_cast<uint8_t*>(dstream);
   sc.info = info;
   sc.n = n; sc.G = G; sc.dim = DIM; sc.eb = sa.eb;
+  const uint32_t nb = static_cast<uint32_t>((G + kScanBlock - 1) / kScanBlock);
   {
     LaunchTimer lt("scan", s);
-    hipLaunchKernelGGL((scan_kernel<T>), dim3(1), dim3(1024), 0, s, sc);
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(256), 0, s, sc);
+    hipLaunchKernelGGL((scan_top_kernel<T>), dim3(1), dim3(256), 0, s, sc, nb);
+    hipLaunchKernelGGL((scan_down_kernel<T>), dim3(nb), dim3(256), 0, s, sc);
   }
 
   CompactArgs ca;
