@@ -440,16 +440,129 @@ struct EncodeArgs {
 __device__ __forceinline__ uint32_t pad_word(uint32_t p) {
   return p + ((p >> 4) << 2);
 }
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1)
+    v |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m));
+  return v;
+}
+
+// Quantised code of one coordinate: sat_u32/sat_u64 of fma(x, 1/(2eb),
+// -min/(2eb)) (src/cudaWarpCompress.cu:179-184), returned modulo 2^32 after
+// the shift by log2(seg): the block id and relative position are masked to
+// 31 bits, so only the low 32 bits of every term matter
+// (src/cudaWarpCompressDouble.cu:186-193 computes them in u64).
+template <typename T>
+__device__ __forceinline__ void quant(T x, T inv, T nmi, uint32_t sh,
+                                      uint32_t* q, uint32_t* qs) {
+  const T f = dfma(x, inv, nmi);
+  if (sizeof(T) == 4) {
+    const uint32_t v = sat_u32(static_cast<float>(f));
+    *q = v;
+    *qs = shr(v, sh);
+  } else {
+    const uint64_t v = sat_u64(static_cast<double>(f));
+    *q = static_cast<uint32_t>(v);
+    *qs = static_cast<uint32_t>(shr(v, sh));
+  }
+}
+
+// One pass of the stable LSD radix sort over a 4-bit digit.  Items are in
+// blocked order (lane l holds positions 16l..16l+15); each lane counts its
+// own items in a private counter column, an exclusive scan over (digit,
+// lane) gives every item its stable rank, and the items are scattered into
+// LDS and read back blocked.
+template <bool kPair>
+__device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[16],
+                                           uint32_t shift, uint32_t lane,
+                                           uint16_t* s_cnt, uint32_t* s_key,
+                                           uint32_t* s_val) {
+  reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(0, 0, 0, 0);
+  reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
+  uint32_t dig[16], lrank[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    dig[k] = (key[k] >> shift) & 15u;
+    const uint32_t idx = dig[k] * 64u + lane;
+    const uint32_t o = s_cnt[idx];
+    s_cnt[idx] = static_cast<uint16_t>(o + 1u);
+    lrank[k] = o;
+  }
+  __syncthreads();
+  {
+    const uint4 c0 = reinterpret_cast<uint4*>(s_cnt)[2 * lane];
+    const uint4 c1 = reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1];
+    uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    uint32_t run = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t lo = w[i] & 0xFFFFu, hi = w[i] >> 16;
+      w[i] = run | ((run + lo) << 16);
+      run += lo + hi;
+    }
+    const uint32_t incl = wave_incl_scan(run, lane);
+    const uint32_t add = incl - run;
+    const uint32_t add2 = add | (add << 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] += add2;
+    reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(w[0], w[1], w[2], w[3]);
+    reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t wi = pad_word(s_cnt[dig[k] * 64u + lane] + lrank[k]);
+    s_key[wi] = key[k];
+    if (kPair) s_val[wi] = val[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q4 = 0; q4 < 4; ++q4) {
+    const uint4 kk = reinterpret_cast<const uint4*>(s_key)[5 * lane + q4];
+    key[4 * q4 + 0] = kk.x; key[4 * q4 + 1] = kk.y;
+    key[4 * q4 + 2] = kk.z; key[4 * q4 + 3] = kk.w;
+    if (kPair) {
+      const uint4 vv = reinterpret_cast<const uint4*>(s_val)[5 * lane + q4];
+      val[4 * q4 + 0] = vv.x; val[4 * q4 + 1] = vv.y;
+      val[4 * q4 + 2] = vv.z; val[4 * q4 + 3] = vv.w;
+    }
+  }
+  __syncthreads();
+}
+
+// MSB-first bit planes of 16 per-lane values (runs 16l..16l+15) stored to
+// dst: plane p (bit nb-1-p) is B bytes at dst + p*B, lane l owns bytes 2l,
+// 2l+1.  The transpose gives each lane a 16-bit word per plane (bit t = run
+// 16l+t); bfrev turns it into the two MSB-first bytes.
+template <int kMaxBits>
+__device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32_t nb,
+                                                 uint32_t B, uint32_t padded,
+                                                 uint8_t* dst, uint32_t lane) {
+  uint32_t t16[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) t16[t] = v[t];
+  transpose16x2(t16);
+#pragma unroll
+  for (int b = 0; b < kMaxBits; ++b) {
+    if (b < static_cast<int>(nb)) {
+      const uint32_t bf = __brev(plane_word(t16, b));
+      uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
+      if (2u * lane < B) d[0] = static_cast<uint8_t>(bf >> 24);
+      if (2u * lane + 1u < B) d[1] = static_cast<uint8_t>(bf >> 16);
+    }
+  }
+  if (lane < padded - B * nb) dst[B * nb + lane] = 0;  // zero pad bytes
+}
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
   using Q = typename Prec<T>::Q;
-  // s_sort holds keys | values during the sort and is reused as the payload
-  // staging buffer afterwards (10240 >= kSlot + 16 bytes).
-  __shared__ __align__(16) uint32_t s_sort[2 * kSortWords];
+  // s_a: sort keys (first 5120 B) and values or relative positions indexed
+  // by blocked position (second 5120 B); after the sort it holds the run
+  // arrays.  s_cnt: radix digit counters.
+  __shared__ __align__(16) uint32_t s_a[2 * kSortWords];
   __shared__ __align__(16) uint16_t s_cnt[16 * 64];
-  __shared__ __align__(16) uint16_t s_rstart[kGroup + 16];
-  __shared__ __align__(16) uint32_t s_rdelta[kGroup];
 
   const uint32_t lane = threadIdx.x;
   const uint64_t c = blockIdx.x;
@@ -459,10 +572,11 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
   const uint64_t base = c * kGroup;
   const uint32_t nvalid =
       static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
+  const bool full = nvalid == kGroup;
   const bool last = (c == a.G - 1);
-  const T* X = static_cast<const T*>(a.x);
-  const T* Y = static_cast<const T*>(a.y);
-  const T* Z = static_cast<const T*>(a.z);
+  const T* X = static_cast<const T*>(a.x) + base;
+  const T* Y = static_cast<const T*>(a.y) + base;
+  const T* Z = static_cast<const T*>(a.z) + base;
 
   // ---- stage 1: load in blocked sort order, group extrema
   T px[16], py[16], pz[16];
@@ -472,19 +586,18 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint32_t j = jbase + 32u * k;
-    px[k] = py[k] = pz[k] = T(0);
-    if (j < nvalid) {
-      px[k] = X[base + j];
-      py[k] = Y[base + j];
-      mn[0] = dmin(mn[0], px[k]);
-      mx[0] = dmax(mx[0], px[k]);
-      mn[1] = dmin(mn[1], py[k]);
-      mx[1] = dmax(mx[1], py[k]);
-      if (DIM == 3) {
-        pz[k] = Z[base + j];
-        mn[2] = dmin(mn[2], pz[k]);
-        mx[2] = dmax(mx[2], pz[k]);
-      }
+    const bool v = full || j < nvalid;
+    px[k] = v ? X[j] : T(0);
+    py[k] = v ? Y[j] : T(0);
+    pz[k] = (DIM == 3 && v) ? Z[j] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t j = jbase + 32u * k;
+    if (full || j < nvalid) {
+      mn[0] = dmin(mn[0], px[k]); mx[0] = dmax(mx[0], px[k]);
+      mn[1] = dmin(mn[1], py[k]); mx[1] = dmax(mx[1], py[k]);
+      if (DIM == 3) { mn[2] = dmin(mn[2], pz[k]); mx[2] = dmax(mx[2], pz[k]); }
     }
   }
 #pragma unroll
@@ -499,112 +612,90 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
   const T seg2eb = static_cast<T>(2u * seg) * eb;
   Q nbx, nby;
   block_counts<T, DIM>(mn[0], mx[0], mn[1], mx[1], seg2eb, &nbx, &nby);
+  const uint32_t nbx32 = static_cast<uint32_t>(nbx);
+  const uint32_t nbxy32 = static_cast<uint32_t>(nbx) * static_cast<uint32_t>(nby);
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
-  const Q m = static_cast<Q>(seg - 1u);
-  const Q sq = static_cast<Q>(seg);
+  const uint32_t m = seg - 1u;
+  const bool pow2 = (seg & m) == 0u;
   const T nmi0 = -mn[0] * inv, nmi1 = -mn[1] * inv, nmi2 = -mn[2] * inv;
+  // 24-bit multiplies are exact when every factor is below 2^24; the
+  // largest shifted code of an axis is the code of its maximum.
+  uint32_t qtop = 0, dummy;
+  {
+    uint32_t qs0, qs1, qs2 = 0;
+    quant<T>(mx[0], inv, nmi0, sh, &dummy, &qs0);
+    quant<T>(mx[1], inv, nmi1, sh, &dummy, &qs1);
+    if (DIM == 3) quant<T>(mx[2], inv, nmi2, sh, &dummy, &qs2);
+    qtop = qs0 | qs1 | qs2 | nbx32 | (DIM == 3 ? nbxy32 : 0u);
+    if (sizeof(T) == 8 && (static_cast<uint64_t>(nbx) >> 32)) qtop = ~0u;
+  }
+  const bool fast24 = qtop < (1u << 24);
   uint32_t key[16], val[16];
-  uint32_t bb = 0, rbl = 0;
+  uint32_t bor = 0, ror = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint32_t j = jbase + 32u * k;
-    if (j < nvalid) {
-      Q qx, qy, qz = 0;
-      if (sizeof(T) == 4) {
-        qx = static_cast<Q>(sat_u32(dfma(px[k], inv, nmi0)));
-        qy = static_cast<Q>(sat_u32(dfma(py[k], inv, nmi1)));
-        if (DIM == 3) qz = static_cast<Q>(sat_u32(dfma(pz[k], inv, nmi2)));
-      } else {
-        qx = static_cast<Q>(sat_u64(static_cast<double>(dfma(px[k], inv, nmi0))));
-        qy = static_cast<Q>(sat_u64(static_cast<double>(dfma(py[k], inv, nmi1))));
-        if (DIM == 3)
-          qz = static_cast<Q>(sat_u64(static_cast<double>(dfma(pz[k], inv, nmi2))));
-      }
-      Q b = shr(qx, sh) + shr(qy, sh) * nbx;
-      Q r = (qx & m) + (qy & m) * sq;
-      if (DIM == 3) {
-        b += shr(qz, sh) * nbx * nby;
-        r += (qz & m) * sq * sq;
-      }
-      key[k] = static_cast<uint32_t>(b & Q(0x7FFFFFFF));
-      val[k] = static_cast<uint32_t>(r & Q(0x7FFFFFFF));
-      bb = max(bb, eff_bits(key[k]));
-      rbl = max(rbl, eff_bits(val[k]));
+    uint32_t qx, qy, qz = 0, sx, sy, sz = 0;
+    quant<T>(px[k], inv, nmi0, sh, &qx, &sx);
+    quant<T>(py[k], inv, nmi1, sh, &qy, &sy);
+    if (DIM == 3) quant<T>(pz[k], inv, nmi2, sh, &qz, &sz);
+    uint32_t b, r;
+    if (fast24) {
+      b = sx + __umul24(sy, nbx32);
+      if (DIM == 3) b += __umul24(sz, nbxy32);
     } else {
-      key[k] = 0xFFFFFFFFu;
-      val[k] = 0xFFFFFFFFu;
-      bb = max(bb, 1u);
-      rbl = max(rbl, 1u);
+      b = sx + sy * nbx32;
+      if (DIM == 3) b += sz * nbxy32;
     }
+    if (pow2) {
+      r = (qx & m) | ((qy & m) << sh);
+      if (DIM == 3) r |= 2u * sh >= 32 ? 0u : (qz & m) << (2u * sh);
+    } else {
+      r = (qx & m) + (qy & m) * seg;
+      if (DIM == 3) r += (qz & m) * seg * seg;
+    }
+    b &= 0x7FFFFFFFu;
+    r &= 0x7FFFFFFFu;
+    const bool v = full || j < nvalid;
+    key[k] = v ? b : 0xFFFFFFFFu;
+    val[k] = v ? r : 0xFFFFFFFFu;
+    bor |= v ? b : 1u;
+    ror |= v ? r : 1u;
   }
-  const uint32_t bidbits = wave_max_u32(bb);
-  const uint32_t rb = wave_max_u32(rbl);
+  const uint32_t bidbits = eff_bits(wave_or(bor));
+  const uint32_t rb = eff_bits(wave_or(ror));
 
-  // ---- stage 3: stable LSD radix sort, 4-bit digits, over [0, sortbits)
-  // Counters are private per lane (column = lane), so counting is serial
-  // and conflict free; an exclusive scan in (digit, lane) order then gives
-  // each item its stable rank.
+  // ---- stage 3: stable sort by block id over [0, sortbits), ties in
+  // blocked position order.  With <= 22 key bits the position rides in the
+  // low 10 bits of one word and the values stay put, indexed by position.
   const uint32_t sortbits = bidbits + (last ? 1u : 0u);
-  uint32_t* s_key = s_sort;
-  uint32_t* s_val = s_sort + kSortWords;
-  for (uint32_t shift = 0; shift < sortbits; shift += 4) {
-    reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(0, 0, 0, 0);
-    reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(0, 0, 0, 0);
-    __syncthreads();
-    uint32_t dig[16], lrank[16];
+  uint32_t* s_key = s_a;
+  uint32_t* s_val = s_a + kSortWords;
+  if (sortbits <= 22) {
+#pragma unroll
+    for (int q4 = 0; q4 < 4; ++q4)
+      reinterpret_cast<uint4*>(s_val)[4 * lane + q4] =
+          make_uint4(val[4 * q4], val[4 * q4 + 1], val[4 * q4 + 2], val[4 * q4 + 3]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) key[k] = (key[k] << 10) | (16u * lane + k);
+    for (uint32_t shift = 10; shift < 10 + sortbits; shift += 4)
+      radix_pass<false>(key, val, shift, lane, s_cnt, s_key, s_val);
+    if (sortbits == 0) __syncthreads();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      dig[k] = (key[k] >> shift) & 15u;
-      const uint32_t idx = dig[k] * 64u + lane;
-      const uint32_t o = s_cnt[idx];
-      s_cnt[idx] = static_cast<uint16_t>(o + 1u);
-      lrank[k] = o;
+      val[k] = s_val[key[k] & 1023u];
+      key[k] >>= 10;
     }
-    __syncthreads();
-    {
-      uint4 c0 = reinterpret_cast<uint4*>(s_cnt)[2 * lane];
-      uint4 c1 = reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1];
-      uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-      uint32_t run = 0;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const uint32_t lo = w[i] & 0xFFFFu, hi = w[i] >> 16;
-        const uint32_t elo = run;
-        run += lo;
-        const uint32_t ehi = run;
-        run += hi;
-        w[i] = elo | (ehi << 16);
-      }
-      const uint32_t incl = wave_incl_scan(run, lane);
-      const uint32_t add = incl - run;
-      const uint32_t add2 = add | (add << 16);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) w[i] += add2;
-      reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(w[0], w[1], w[2], w[3]);
-      reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(w[4], w[5], w[6], w[7]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t r = s_cnt[dig[k] * 64u + lane] + lrank[k];
-      const uint32_t wi = pad_word(r);
-      s_key[wi] = key[k];
-      s_val[wi] = val[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q4 = 0; q4 < 4; ++q4) {
-      const uint4 kk = reinterpret_cast<const uint4*>(s_key)[5 * lane + q4];
-      const uint4 vv = reinterpret_cast<const uint4*>(s_val)[5 * lane + q4];
-      key[4 * q4 + 0] = kk.x; key[4 * q4 + 1] = kk.y;
-      key[4 * q4 + 2] = kk.z; key[4 * q4 + 3] = kk.w;
-      val[4 * q4 + 0] = vv.x; val[4 * q4 + 1] = vv.y;
-      val[4 * q4 + 2] = vv.z; val[4 * q4 + 3] = vv.w;
-    }
-    __syncthreads();
+    // invalid ranks keep all-ones block ids (never compared below)
+  } else {
+    for (uint32_t shift = 0; shift < sortbits; shift += 4)
+      radix_pass<true>(key, val, shift, lane, s_cnt, s_key, s_val);
   }
+  __syncthreads();
 
   // ---- stage 4a: runs of equal block id (lane l holds ranks 16l..16l+15)
+  uint16_t* s_rstart = reinterpret_cast<uint16_t*>(s_a);          // [1040]
+  uint32_t* s_rdelta = s_a + 520;                                 // [1024]
   const uint32_t prev0 =
       static_cast<uint32_t>(__shfl_up(static_cast<int>(key[15]), 1));
   uint32_t fl = 0;
@@ -636,7 +727,7 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
       static_cast<uint32_t>(__shfl(static_cast<int>(key[0]), 0));
   __syncthreads();
   uint32_t del[16], cnt[16];
-  uint32_t dbl = 0, cbl = 0;
+  uint32_t dor = 0, cor = 0;
 #pragma unroll
   for (int t = 0; t < 16; ++t) {
     const uint32_t i = 16u * lane + t;
@@ -645,69 +736,36 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
     if (i < U) {
       del[t] = s_rdelta[i];
       cnt[t] = static_cast<uint32_t>(s_rstart[i + 1]) - s_rstart[i];
-      dbl = max(dbl, eff_bits(del[t]));
-      cbl = max(cbl, eff_bits(cnt[t]));
     }
+    dor |= del[t];
+    cor |= cnt[t];
   }
-  const uint32_t db = wave_max_u32(dbl);
-  const uint32_t cb = wave_max_u32(cbl);
+  const uint32_t db = eff_bits(wave_or(dor));
+  const uint32_t cb = eff_bits(wave_or(cor));
   uint32_t D, C, R;
   section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
   const uint32_t loc = D + C + R;
   const uint32_t B = (U + 7u) >> 3;
 
-  // ---- stage 4b: bit planes into the LDS staging buffer
-  uint8_t* stg = reinterpret_cast<uint8_t*>(s_sort);
-  const uint32_t loc16 = (loc + 15u) & ~15u;
-  for (uint32_t o = 16u * lane; o < loc16; o += 1024u)
-    *reinterpret_cast<uint4*>(stg + o) = make_uint4(0, 0, 0, 0);
-  __syncthreads();
-  // Delta and count planes: transpose the lane's 16 runs so that each
-  // 16-bit plane word holds runs 16l..16l+15 (bit t = run 16l+t); bfrev
-  // turns it into the two MSB-first bytes 2l, 2l+1 of the plane.
-  {
-    uint32_t t16[16];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) t16[t] = del[t];
-    transpose16x2(t16);
-#pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      if (b < static_cast<int>(db)) {
-        const uint32_t bf = __brev(plane_word(t16, b));
-        uint8_t* dst = stg + (db - 1u - b) * B + 2u * lane;
-        if (2u * lane < B) dst[0] = static_cast<uint8_t>(bf >> 24);
-        if (2u * lane + 1u < B) dst[1] = static_cast<uint8_t>(bf >> 16);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 16; ++t) t16[t] = cnt[t];
-    transpose16x2(t16);
-#pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      if (b < static_cast<int>(cb)) {
-        const uint32_t bf = __brev(plane_word(t16, b));
-        uint8_t* dst = stg + D + (cb - 1u - b) * B + 2u * lane;
-        if (2u * lane < B) dst[0] = static_cast<uint8_t>(bf >> 24);
-        if (2u * lane + 1u < B) dst[1] = static_cast<uint8_t>(bf >> 16);
-      }
-    }
-  }
-  // Rel planes: LSB-first over rank, so the plane word is stored as is.
+  // ---- stage 4b: bit planes straight into the group's slot
+  uint8_t* slot = a.slots + c * static_cast<uint64_t>(kSlot);
+  store_msb_planes<32>(del, db, B, D, slot, lane);
+  store_msb_planes<16>(cnt, cb, B, C, slot + D, lane);
   {
     uint32_t t16[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) t16[k] = val[k];
     transpose16x2(t16);
+    uint8_t* pr = slot + D + C;
 #pragma unroll
     for (int b = 0; b < 32; ++b) {
       if (b < static_cast<int>(rb))
-        *reinterpret_cast<uint16_t*>(stg + D + C + (rb - 1u - b) * 128u + 2u * lane) =
+        *reinterpret_cast<uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
             static_cast<uint16_t>(plane_word(t16, b));
     }
   }
-  __syncthreads();
 
-  // ---- meta, size, payload
+  // ---- meta and size
   {
     constexpr uint32_t M = Prec<T>::kMeta;
     uint32_t word = 0;
@@ -728,13 +786,10 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
         const uint32_t e = lane >> 1;
         const T v = e == 0 ? mn[0] : e == 1 ? mn[1] : e == 2 ? mn[2]
                   : e == 3 ? mx[0] : e == 4 ? mx[1] : mx[2];
-        const double d = static_cast<double>(v);
-        const uint64_t u = static_cast<uint64_t>(__double_as_longlong(d));
+        const uint64_t u = static_cast<uint64_t>(__double_as_longlong(static_cast<double>(v)));
         word = (lane & 1) ? static_cast<uint32_t>(u >> 32) : static_cast<uint32_t>(u);
       } else if (lane == 12) {
         word = outlier;
-      } else if (lane == 13) {
-        word = 0;
       } else if (lane == 14) {
         word = db | (cb << 16);
       } else if (lane == 15) {
@@ -745,9 +800,6 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
       reinterpret_cast<uint32_t*>(a.meta + c * M)[lane] = word;
   }
   if (lane == 0) a.loc[c] = loc;
-  uint8_t* slot = a.slots + c * static_cast<uint64_t>(kSlot);
-  for (uint32_t o = 16u * lane; o < loc16; o += 1024u)
-    *reinterpret_cast<uint4*>(slot + o) = *reinterpret_cast<const uint4*>(stg + o);
 }
 
 // ------------------------------------------------- offsets scan + header
@@ -906,9 +958,12 @@ template <typename T, int DIM>
 __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   using Q = typename Prec<T>::Q;
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  // Staged payload; once its planes are in registers the same buffer holds
+  // the block id of every run and the run-start flags of every rank.
   __shared__ __align__(16) uint32_t s_pay[kSlot / 4];
-  __shared__ __align__(16) Q s_bid[kGroup];
-  __shared__ __align__(16) uint8_t s_start[kGroup];
+  Q* s_bid = reinterpret_cast<Q*>(s_pay);
+  uint8_t* s_start = reinterpret_cast<uint8_t*>(s_pay) + kGroup * sizeof(Q);
+  static_assert(kGroup * sizeof(Q) + kGroup <= kSlot, "run tables fit the stage");
 
   const uint32_t lane = threadIdx.x;
   const uint64_t c = blockIdx.x;
@@ -949,7 +1004,6 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
       }
     }
   }
-  reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
   __syncthreads();
   const uint8_t* pb = reinterpret_cast<const uint8_t*>(s_pay);
   const uint32_t B = (U + 7u) >> 3;
@@ -997,6 +1051,26 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
 #pragma unroll
     for (int t = 0; t < 16; ++t) cnt[t] = t16[t];
   }
+  // ---- relative positions of ranks 16l..16l+15 (LSB-first planes)
+  uint32_t rel[16];
+  {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) rel[k] = 0;
+    const uint8_t* pr = pb + D + C;
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      if (b < static_cast<int>(rb)) {
+        const uint32_t w =
+            *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
+        rel[b & 15] |= b < 16 ? w : (w << 16);
+      }
+    }
+    transpose16x2(rel);
+  }
+  // every plane is in registers: the stage buffer becomes the run tables
+  __syncthreads();
+  reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
+  __syncthreads();
   // block id per run = outlier + prefix of deltas (delta of run 0 ignored,
   // src/cudaWarpCompress.cu:930-935); run end = prefix of counts.
   Q bsum = 0;
@@ -1025,22 +1099,6 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
       const uint32_t st = cend[t] + cpre - cnt[t];
       if (st < kGroup) s_start[st] = 1;
     }
-  }
-  // ---- relative positions of ranks 16l..16l+15 (LSB-first planes)
-  uint32_t rel[16];
-  {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) rel[k] = 0;
-    const uint8_t* pr = pb + D + C;
-#pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      if (b < static_cast<int>(rb)) {
-        const uint32_t w =
-            *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
-        rel[b & 15] |= b < 16 ? w : (w << 16);
-      }
-    }
-    transpose16x2(rel);
   }
   __syncthreads();
   // ---- rank -> run index by a prefix count of run starts
