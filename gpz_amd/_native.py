@@ -35,7 +35,7 @@ ERRORS = {
 EXPORTS = (
     "gpz_version", "gpz_stream_bound", "gpz_compress_workspace_bytes",
     "gpz_compress", "gpz_compress_result", "gpz_parse_header",
-    "gpz_read_header", "gpz_decompress", "gpz_min_max_workspace_bytes",
+    "gpz_read_header", "gpz_stream_bytes", "gpz_decompress", "gpz_min_max_workspace_bytes",
     "gpz_min_max", "gpz_rel_to_abs", "gpz_timing_enable", "gpz_timing_reset",
     "gpz_timing_read",
 )
@@ -84,6 +84,7 @@ def _bind(L):
         "gpz_compress_result": ([vp, P(Result), vp], i32),
         "gpz_parse_header": ([vp, u64, P(Header)], i32),
         "gpz_read_header": ([vp, P(Header), vp], i32),
+        "gpz_stream_bytes": ([P(Header), vp, P(u64), vp], i32),
         "gpz_decompress": ([P(Header), vp, u64, vp, vp, vp, vp], i32),
         "gpz_min_max_workspace_bytes": ([], u64),
         "gpz_min_max": ([u32, u32, u64, vp, vp, vp, vp, vp, vp], i32),

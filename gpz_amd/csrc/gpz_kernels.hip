@@ -269,21 +269,20 @@ __device__ __forceinline__ void block_counts(T mn0, T mx0, T mn1, T mx1,
 
 // ------------------------------------------------------- workspace layout
 struct WsLayout {
-  uint64_t info, win, loc, off, bsum, meta, slots, total;
+  uint64_t info, win, loc, off, bsum, slots, total;
 };
 __host__ __device__ inline uint64_t align256(uint64_t v) {
   return (v + 255) & ~255ull;
 }
 inline WsLayout ws_layout(uint32_t bits, uint64_t G) {
-  const uint64_t M = bits == 64 ? 64 : 36;
+  (void)bits;
   WsLayout w;
   w.info = 0;
   w.win = 256;
   w.loc = align256(w.win + 6ull * kSampleBlocks * 8);
   w.off = align256(w.loc + 8 * G);
   w.bsum = align256(w.off + 8 * G);
-  w.meta = align256(w.bsum + 8 * ((G + 1023) / 1024));
-  w.slots = align256(w.meta + M * G);
+  w.slots = align256(w.bsum + 8 * ((G + 1023) / 1024));
   w.total = align256(w.slots + static_cast<uint64_t>(kSlot) * G);
   return w;
 }
@@ -432,7 +431,7 @@ struct EncodeArgs {
   uint64_t n, G;
   double eb;
   const DevInfo* info;
-  uint8_t* meta;    // [G * kMeta]
+  uint8_t* meta;    // stream meta table [G * kMeta]
   uint64_t* loc;    // [G]
   uint8_t* slots;   // [G * kSlot]
 };
@@ -452,12 +451,20 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
 // the shift by log2(seg): the block id and relative position are masked to
 // 31 bits, so only the low 32 bits of every term matter
 // (src/cudaWarpCompressDouble.cu:186-193 computes them in u64).
+// v_cvt_u32_f32 truncates, clamps to [0, 2^32-1] and maps NaN to 0, the
+// same result as CUDA's cvt.rzi.u32.f32 and as sat_u32 above.
+__device__ __forceinline__ uint32_t cvt_u32_sat(float f) {
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
+  return r;
+}
+
 template <typename T>
 __device__ __forceinline__ void quant(T x, T inv, T nmi, uint32_t sh,
                                       uint32_t* q, uint32_t* qs) {
   const T f = dfma(x, inv, nmi);
   if (sizeof(T) == 4) {
-    const uint32_t v = sat_u32(static_cast<float>(f));
+    const uint32_t v = cvt_u32_sat(static_cast<float>(f));
     *q = v;
     *qs = shr(v, sh);
   } else {
@@ -913,28 +920,42 @@ __global__ __launch_bounds__(256) void scan_down_kernel(ScanArgs a) {
 struct CompactArgs {
   const uint64_t* loc;
   const uint64_t* off;
-  const uint8_t* meta;
   const uint8_t* slots;
   uint8_t* stream;
   const DevInfo* info;
   uint64_t G;
 };
 
+// Payload compaction (src/cudaWarpCompress.cu:32-56): every wave copies
+// whole groups from their fixed slots to their packed offsets, grid-stride,
+// eight dword loads in flight per lane.
 template <typename T>
 __global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
   if (a.info->status != GPZ_OK) return;
-  const uint64_t c = blockIdx.x;
-  const uint32_t t = threadIdx.x;
-  if (t < M / 4)
-    reinterpret_cast<uint32_t*>(a.stream + P + 8 * a.G + M * c)[t] =
-        reinterpret_cast<const uint32_t*>(a.meta + M * c)[t];
-  const uint32_t words = static_cast<uint32_t>(a.loc[c]) >> 2;
-  const uint32_t* src =
-      reinterpret_cast<const uint32_t*>(a.slots + c * static_cast<uint64_t>(kSlot));
-  uint32_t* dst = reinterpret_cast<uint32_t*>(a.stream + P + (8ull + M) * a.G +
-                                              a.off[c]);
-  for (uint32_t i = t; i < words; i += 256) dst[i] = src[i];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
+  const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * 4;
+  uint8_t* pay = a.stream + P + (8ull + M) * a.G;
+  for (uint64_t c = wave; c < a.G; c += nwaves) {
+    const uint32_t words = static_cast<uint32_t>(a.loc[c]) >> 2;
+    const uint32_t* src =
+        reinterpret_cast<const uint32_t*>(a.slots + c * static_cast<uint64_t>(kSlot));
+    uint32_t* dst = reinterpret_cast<uint32_t*>(pay + a.off[c]);
+    for (uint32_t i0 = 0; i0 < words; i0 += 512) {
+      uint32_t v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t i = i0 + 64u * k + lane;
+        v[k] = i < words ? src[i] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t i = i0 + 64u * k + lane;
+        if (i < words) dst[i] = v[k];
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------- decoder
@@ -960,10 +981,15 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
   // Staged payload; once its planes are in registers the same buffer holds
   // the block id of every run and the run-start flags of every rank.
+  // FP32 groups with at most 512 runs keep the block coordinates of each
+  // run (already multiplied by seg) instead, so that points need no
+  // division.
   __shared__ __align__(16) uint32_t s_pay[kSlot / 4];
   Q* s_bid = reinterpret_cast<Q*>(s_pay);
-  uint8_t* s_start = reinterpret_cast<uint8_t*>(s_pay) + kGroup * sizeof(Q);
-  static_assert(kGroup * sizeof(Q) + kGroup <= kSlot, "run tables fit the stage");
+  uint4* s_tab = reinterpret_cast<uint4*>(s_pay);
+  uint8_t* s_start = reinterpret_cast<uint8_t*>(s_pay) + 8192;
+  static_assert(kGroup * sizeof(Q) <= 8192 && 512 * 16 <= 8192 &&
+                8192 + kGroup <= kSlot, "run tables fit the stage");
 
   const uint32_t lane = threadIdx.x;
   const uint64_t c = blockIdx.x;
@@ -1071,6 +1097,14 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   __syncthreads();
   reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
   __syncthreads();
+  const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
+  const Q m = static_cast<Q>(seg - 1u);
+  const Q sq = static_cast<Q>(seg);
+  const Q nxy = static_cast<Q>(nbx * nby);
+  const Div32 dvx = div32_make(static_cast<uint32_t>(nbx ? nbx : 1));
+  const Div32 dvy = div32_make(static_cast<uint32_t>(nby ? nby : 1));
+  const Div32 dvxy = div32_make(static_cast<uint32_t>(nxy ? nxy : 1));
+  const bool table = sizeof(T) == 4 && U <= 512;
   // block id per run = outlier + prefix of deltas (delta of run 0 ignored,
   // src/cudaWarpCompress.cu:930-935); run end = prefix of counts.
   Q bsum = 0;
@@ -1095,7 +1129,17 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   for (int t = 0; t < 16; ++t) {
     const uint32_t i = 16u * lane + t;
     if (i < U) {
-      s_bid[i] = static_cast<Q>(bid[t] + bpre);
+      const Q b = static_cast<Q>(bid[t] + bpre);
+      if (table) {
+        const Q qb = qdiv<Q>(b, nbx, dvx);
+        const Q bx = nbx ? b - qb * nbx : b;
+        const Q by = nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb;
+        const Q bz = qdiv<Q>(b, nxy, dvxy);
+        s_tab[i] = make_uint4(static_cast<uint32_t>(bx * sq), static_cast<uint32_t>(by * sq),
+                              static_cast<uint32_t>(bz * sq), 0u);
+      } else {
+        s_bid[i] = b;
+      }
       const uint32_t st = cend[t] + cpre - cnt[t];
       if (st < kGroup) s_start[st] = 1;
     }
@@ -1113,33 +1157,31 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   }
   const uint32_t rpre = wave_incl_scan(rs, lane) - rs;
 
-  const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
-  const Q m = static_cast<Q>(seg - 1u);
-  const Q sq = static_cast<Q>(seg);
-  const Q nxy = static_cast<Q>(nbx * nby);
-  const Div32 dvx = div32_make(static_cast<uint32_t>(nbx ? nbx : 1));
-  const Div32 dvy = div32_make(static_cast<uint32_t>(nby ? nby : 1));
-  const Div32 dvxy = div32_make(static_cast<uint32_t>(nxy ? nxy : 1));
   T ox[16], oy[16], oz[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     uint32_t run = runof[k] + rpre;
     run = run ? run - 1u : 0u;
     if (run >= U) run = U - 1u;
-    const Q b = s_bid[run];
-    const Q qb = qdiv<Q>(b, nbx, dvx);            // b / nbx
-    const Q bx = nbx ? b - qb * nbx : b;          // b % nbx
-    const Q by = nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb;
     const Q rl = static_cast<Q>(rel[k]);
-    const Q qx = bx * sq + (rl & m);
-    const Q qy = by * sq + (shr(rl, sh) & m);
+    Q qx, qy, qz = 0;
+    if (table) {
+      const uint4 tb = s_tab[run];
+      qx = static_cast<Q>(tb.x) + (rl & m);
+      qy = static_cast<Q>(tb.y) + (shr(rl, sh) & m);
+      if (DIM == 3) qz = static_cast<Q>(tb.z) + shr(rl, 2u * sh);
+    } else {
+      const Q b = s_bid[run];
+      const Q qb = qdiv<Q>(b, nbx, dvx);            // b / nbx
+      const Q bx = nbx ? b - qb * nbx : b;          // b % nbx
+      const Q by = nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb;
+      qx = bx * sq + (rl & m);
+      qy = by * sq + (shr(rl, sh) & m);
+      if (DIM == 3) qz = qdiv<Q>(b, nxy, dvxy) * sq + shr(rl, 2u * sh);
+    }
     ox[k] = dfma(static_cast<T>((qx << 1) | 1u), eb, mm[0]);
     oy[k] = dfma(static_cast<T>((qy << 1) | 1u), eb, mm[1]);
-    if (DIM == 3) {
-      const Q bz = qdiv<Q>(b, nxy, dvxy);
-      const Q qz = bz * sq + shr(rl, 2u * sh);
-      oz[k] = dfma(static_cast<T>((qz << 1) | 1u), eb, mm[2]);
-    }
+    if (DIM == 3) oz[k] = dfma(static_cast<T>((qz << 1) | 1u), eb, mm[2]);
   }
   // ---- store ranks 16l..16l+15 of each plane
   T* X = static_cast<T*>(a.x) + base + 16u * lane;
@@ -1265,7 +1307,7 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   ea.x = dx; ea.y = dy; ea.z = dz;
   ea.n = n; ea.G = G; ea.eb = sa.eb;
   ea.info = info;
-  ea.meta = ws + L.meta;
+  ea.meta = static_cast<uint8_t*>(dstream) + HostPrec<T>::P + 8 * G;
   ea.loc = reinterpret_cast<uint64_t*>(ws + L.loc);
   ea.slots = ws + L.slots;
   {
@@ -1291,14 +1333,14 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   CompactArgs ca;
   ca.loc = ea.loc;
   ca.off = sc.off;
-  ca.meta = ea.meta;
   ca.slots = ea.slots;
   ca.stream = sc.stream;
   ca.info = info;
   ca.G = G;
   {
     LaunchTimer lt("compact", s);
-    hipLaunchKernelGGL((compact_kernel<T>), dim3(static_cast<uint32_t>(G)), dim3(256), 0, s, ca);
+    const uint64_t wgs = (G + 3) / 4 < 4096 ? (G + 3) / 4 : 4096;
+    hipLaunchKernelGGL((compact_kernel<T>), dim3(static_cast<uint32_t>(wgs)), dim3(256), 0, s, ca);
   }
   return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
 }
@@ -1431,6 +1473,32 @@ int gpz_read_header(const void* d_stream, gpz_header* out, gpz_stream_t stream) 
       hipStreamSynchronize(s) != hipSuccess)
     return GPZ_ERR_HIP;
   return gpz_parse_header(h, 32, out);
+}
+
+int gpz_stream_bytes(const gpz_header* h, const void* d_stream,
+                     uint64_t* bytes, gpz_stream_t stream) {
+  if (!h || !d_stream || !bytes || h->groups == 0) return GPZ_ERR_INVALID_ARG;
+  const uint64_t M = h->float_bits == 64 ? 64 : 36;
+  const uint64_t G = h->groups;
+  const uint8_t* b = static_cast<const uint8_t*>(d_stream);
+  uint64_t off = 0;
+  uint8_t meta[64];
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(&off, b + h->header_bytes + 8 * (G - 1), 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(meta, b + h->header_bytes + 8 * G + M * (G - 1), M, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPZ_ERR_HIP;
+  uint16_t u[4];
+  memcpy(u, meta + (M == 64 ? 56 : 28), 8);
+  uint32_t D, C, R;
+  if (h->float_bits == 64) {
+    if (h->dim == 3) section_sizes<double, 3>(u[3], u[0], u[1], u[2], &D, &C, &R);
+    else section_sizes<double, 2>(u[3], u[0], u[1], u[2], &D, &C, &R);
+  } else {
+    section_sizes<float, 3>(u[3], u[0], u[1], u[2], &D, &C, &R);
+  }
+  *bytes = h->min_stream_bytes + off + D + C + R;
+  return GPZ_OK;
 }
 
 int gpz_decompress(const gpz_header* h, const void* d_stream,

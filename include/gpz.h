@@ -109,6 +109,11 @@ int gpz_parse_header(const void* h_stream, uint64_t bytes, gpz_header* out);
  * src/cudaWarpCompress.cu:1807-1810). */
 int gpz_read_header(const void* d_stream, gpz_header* out, gpz_stream_t stream);
 
+/* Total size of a device-resident stream, from its header, the last
+ * group's offset and meta (synchronises the stream). */
+int gpz_stream_bytes(const gpz_header* h, const void* d_stream,
+                     uint64_t* bytes, gpz_stream_t stream);
+
 /* Decompress a device stream into SoA planes d_x, d_y[, d_z] (N elements
  * each; d_z untouched for dim 2).  Point 1024*c + r receives the point of
  * sorted rank r of group c, as in the reference decoder
