@@ -65,13 +65,11 @@ def build_datagen():
 
 def build_cli():
     """The `gpz` drop-in CLI and the C++ ICudaCompressor classes."""
-    srcs = [os.path.join(CSRC, f) for f in ("gpz_cxx.cpp", "gpz_cli.cpp")]
-    if not all(os.path.exists(s) for s in srcs):
-        return None
     out = os.path.join(LIB, "gpz")
     tmp = out + ".tmp"
-    _run([HIPCC, "-O2", "-std=c++17", f"-I{INCLUDE}", "-o", tmp, *srcs,
-          f"-L{LIB}", "-lgpz_amd", f"-Wl,-rpath,{LIB}", "-Wl,-rpath,$ORIGIN"])
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "gpz_cxx.cpp"),
+          os.path.join(CSRC, "gpz_cli.hip"), f"-L{LIB}", "-lgpz_amd",
+          "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, out)
     return out
 
