@@ -108,8 +108,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        sys.exit("bench.py runs on one GPU: this build has no multi-GPU path "
-                 "(the compressor shards by 1024-point groups; see DESIGN.md)")
+        sys.exit("bench.py runs on one GPU only: there is no multi-GPU path "
+                 "(the compressor would shard by contiguous ranges of "
+                 "1024-point groups, exchanging one 8-byte total per GPU)")
 
     import numpy as np
     import torch
