@@ -139,6 +139,17 @@ __device__ __forceinline__ double dfma(double a, double b, double c) {
 __device__ __forceinline__ float dceil(float a) { return ceilf(a); }
 __device__ __forceinline__ double dceil(double a) { return ceil(a); }
 
+// LDS ordering inside a single-wave workgroup.  A wave's LDS instructions
+// execute in order, so a wavefront-scope fence plus wave_barrier (a code
+// motion barrier) is enough; unlike __syncthreads it does not wait for
+// outstanding global loads, which keeps prefetches in flight.  Only valid
+// for kernels launched with 64-thread workgroups.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // wave64 collectives (ds_bpermute based shuffles)
 template <typename V> __device__ __forceinline__ V wave_min(V v) {
 #pragma unroll
@@ -486,7 +497,7 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
                                            uint32_t* s_val) {
   reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(0, 0, 0, 0);
   reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(0, 0, 0, 0);
-  __syncthreads();
+  wave_sync();
   uint32_t dig[16], lrank[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
@@ -496,7 +507,7 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
     s_cnt[idx] = static_cast<uint16_t>(o + 1u);
     lrank[k] = o;
   }
-  __syncthreads();
+  wave_sync();
   {
     const uint4 c0 = reinterpret_cast<uint4*>(s_cnt)[2 * lane];
     const uint4 c1 = reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1];
@@ -516,14 +527,14 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
     reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(w[0], w[1], w[2], w[3]);
     reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(w[4], w[5], w[6], w[7]);
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint32_t wi = pad_word(s_cnt[dig[k] * 64u + lane] + lrank[k]);
     s_key[wi] = key[k];
     if (kPair) s_val[wi] = val[k];
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int q4 = 0; q4 < 4; ++q4) {
     const uint4 kk = reinterpret_cast<const uint4*>(s_key)[5 * lane + q4];
@@ -535,7 +546,7 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
       val[4 * q4 + 2] = vv.z; val[4 * q4 + 3] = vv.w;
     }
   }
-  __syncthreads();
+  wave_sync();
 }
 
 // MSB-first bit planes of 16 per-lane values (runs 16l..16l+15) stored to
@@ -562,33 +573,19 @@ __device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32
   if (lane < padded - B * nb) dst[B * nb + lane] = 0;  // zero pad bytes
 }
 
+// Coordinates of group c in blocked sort order (lane l, item k = slot
+// 32*(16*(l&1) + k) + (l>>1)); slots past N read 0.
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
-  using Q = typename Prec<T>::Q;
-  // s_a: sort keys (first 5120 B) and values or relative positions indexed
-  // by blocked position (second 5120 B); after the sort it holds the run
-  // arrays.  s_cnt: radix digit counters.
-  __shared__ __align__(16) uint32_t s_a[2 * kSortWords];
-  __shared__ __align__(16) uint16_t s_cnt[16 * 64];
-
-  const uint32_t lane = threadIdx.x;
-  const uint64_t c = blockIdx.x;
-  if (a.info->status != GPZ_OK) return;
-  const uint32_t seg = a.info->seg_size;
-  const T eb = static_cast<T>(a.eb);
+__device__ __forceinline__ void load_group(const EncodeArgs& a, uint64_t c,
+                                           uint32_t lane, T (&px)[16],
+                                           T (&py)[16], T (&pz)[16]) {
   const uint64_t base = c * kGroup;
   const uint32_t nvalid =
       static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
   const bool full = nvalid == kGroup;
-  const bool last = (c == a.G - 1);
   const T* X = static_cast<const T*>(a.x) + base;
   const T* Y = static_cast<const T*>(a.y) + base;
   const T* Z = static_cast<const T*>(a.z) + base;
-
-  // ---- stage 1: load in blocked sort order, group extrema
-  T px[16], py[16], pz[16];
-  T mn[3] = {Prec<T>::lo(), Prec<T>::lo(), Prec<T>::lo()};
-  T mx[3] = {-Prec<T>::lo(), -Prec<T>::lo(), -Prec<T>::lo()};
   const uint32_t jbase = 512u * (lane & 1u) + (lane >> 1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
@@ -598,6 +595,27 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
     py[k] = v ? Y[j] : T(0);
     pz[k] = (DIM == 3 && v) ? Z[j] : T(0);
   }
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
+                                             uint32_t lane, const T (&px)[16],
+                                             const T (&py)[16],
+                                             const T (&pz)[16], uint32_t* s_a,
+                                             uint16_t* s_cnt) {
+  using Q = typename Prec<T>::Q;
+  const uint32_t seg = a.info->seg_size;
+  const T eb = static_cast<T>(a.eb);
+  const uint64_t base = c * kGroup;
+  const uint32_t nvalid =
+      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
+  const bool full = nvalid == kGroup;
+  const bool last = (c == a.G - 1);
+  const uint32_t jbase = 512u * (lane & 1u) + (lane >> 1);
+
+  // ---- stage 1: group extrema
+  T mn[3] = {Prec<T>::lo(), Prec<T>::lo(), Prec<T>::lo()};
+  T mx[3] = {-Prec<T>::lo(), -Prec<T>::lo(), -Prec<T>::lo()};
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint32_t j = jbase + 32u * k;
@@ -687,7 +705,7 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
     for (int k = 0; k < 16; ++k) key[k] = (key[k] << 10) | (16u * lane + k);
     for (uint32_t shift = 10; shift < 10 + sortbits; shift += 4)
       radix_pass<false>(key, val, shift, lane, s_cnt, s_key, s_val);
-    if (sortbits == 0) __syncthreads();
+    if (sortbits == 0) wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       val[k] = s_val[key[k] & 1023u];
@@ -698,7 +716,7 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
     for (uint32_t shift = 0; shift < sortbits; shift += 4)
       radix_pass<true>(key, val, shift, lane, s_cnt, s_key, s_val);
   }
-  __syncthreads();
+  wave_sync();
 
   // ---- stage 4a: runs of equal block id (lane l holds ranks 16l..16l+15)
   uint16_t* s_rstart = reinterpret_cast<uint16_t*>(s_a);          // [1040]
@@ -732,7 +750,7 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
   }
   const uint32_t outlier =
       static_cast<uint32_t>(__shfl(static_cast<int>(key[0]), 0));
-  __syncthreads();
+  wave_sync();
   uint32_t del[16], cnt[16];
   uint32_t dor = 0, cor = 0;
 #pragma unroll
@@ -807,6 +825,24 @@ __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
       reinterpret_cast<uint32_t*>(a.meta + c * M)[lane] = word;
   }
   if (lane == 0) a.loc[c] = loc;
+}
+
+// One wave per group.  (A variant that prefetched the next group's
+// coordinates into registers measured slower: the doubled register set
+// halves occupancy.)
+template <typename T, int DIM>
+__global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
+  // s_a: sort keys (first 5120 B) and values or relative positions indexed
+  // by blocked position (second 5120 B); after the sort it holds the run
+  // arrays.  s_cnt: radix digit counters.
+  __shared__ __align__(16) uint32_t s_a[2 * kSortWords];
+  __shared__ __align__(16) uint16_t s_cnt[16 * 64];
+  const uint32_t lane = threadIdx.x;
+  if (a.info->status != GPZ_OK) return;
+  const uint64_t c = blockIdx.x;
+  T px[16], py[16], pz[16];
+  load_group<T, DIM>(a, c, lane, px, py, pz);
+  encode_group<T, DIM>(a, c, lane, px, py, pz, s_a, s_cnt);
 }
 
 // ------------------------------------------------- offsets scan + header
@@ -1030,7 +1066,7 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
       }
     }
   }
-  __syncthreads();
+  wave_sync();
   const uint8_t* pb = reinterpret_cast<const uint8_t*>(s_pay);
   const uint32_t B = (U + 7u) >> 3;
 
@@ -1094,9 +1130,9 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
     transpose16x2(rel);
   }
   // every plane is in registers: the stage buffer becomes the run tables
-  __syncthreads();
+  wave_sync();
   reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
-  __syncthreads();
+  wave_sync();
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
   const Q m = static_cast<Q>(seg - 1u);
   const Q sq = static_cast<Q>(seg);
@@ -1144,7 +1180,7 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
       if (st < kGroup) s_start[st] = 1;
     }
   }
-  __syncthreads();
+  wave_sync();
   // ---- rank -> run index by a prefix count of run starts
   const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
   const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
@@ -1255,6 +1291,7 @@ template <> struct HostPrec<double> { static constexpr uint32_t P = 32, M = 64; 
 
 inline uint64_t groups_of(uint64_t n) { return (n + kGroup - 1) / kGroup; }
 
+
 bool params_ok(const gpz_params* p) {
   if (!p) return false;
   if (p->dim != 2 && p->dim != 3) return false;
@@ -1312,7 +1349,8 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   ea.slots = ws + L.slots;
   {
     LaunchTimer lt("encode", s);
-    hipLaunchKernelGGL((encode_kernel<T, DIM>), dim3(static_cast<uint32_t>(G)), dim3(64), 0, s, ea);
+    hipLaunchKernelGGL((encode_kernel<T, DIM>), dim3(static_cast<uint32_t>(G)),
+                       dim3(64), 0, s, ea);
   }
 
   ScanArgs sc;
