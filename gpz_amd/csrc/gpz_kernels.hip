@@ -206,6 +206,31 @@ __device__ __forceinline__ void transpose16x2(uint32_t (&a)[16]) {
     }
   }
 }
+// 16x16 bit transpose of 16-bit values held two per register: on entry
+// r[i] = item i | item i+8 << 16; on exit r[i] bits 0..15 hold bit i of
+// items 0..15 and bits 16..31 hold bit i+8.  The first stage (swap of the
+// off-diagonal 8x8 blocks) is one byte permutation per register.  Also its
+// own inverse.
+__device__ __forceinline__ void transpose16_packed(uint32_t (&r)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(r[i], r[i], 0x03010200u);
+#pragma unroll
+  for (int st = 0; st < 3; ++st) {
+    const int j = 4 >> st;
+    const uint32_t m = st == 0 ? 0x0F0F0F0Fu : st == 1 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & j) continue;
+      const uint32_t t = ((r[k] >> j) ^ r[k + j]) & m;
+      r[k + j] ^= t;
+      r[k] ^= t << j;
+    }
+  }
+}
+__device__ __forceinline__ uint32_t plane_word16(const uint32_t (&r)[8], int b) {
+  return b < 8 ? (r[b] & 0xFFFFu) : (r[b - 8] >> 16);
+}
+
 // Word of bit plane b (0..31) after transpose16x2.
 __device__ __forceinline__ uint32_t plane_word(const uint32_t (&a)[16], int b) {
   return b < 16 ? (a[b] & 0xFFFFu) : (a[b - 16] >> 16);
@@ -477,11 +502,11 @@ __device__ __forceinline__ void quant(T x, T inv, T nmi, uint32_t sh,
   if (sizeof(T) == 4) {
     const uint32_t v = cvt_u32_sat(static_cast<float>(f));
     *q = v;
-    *qs = shr(v, sh);
+    *qs = v >> sh;  // sh = ctz(seg) <= 31
   } else {
     const uint64_t v = sat_u64(static_cast<double>(f));
     *q = static_cast<uint32_t>(v);
-    *qs = static_cast<uint32_t>(shr(v, sh));
+    *qs = static_cast<uint32_t>(v >> sh);
   }
 }
 
@@ -557,17 +582,33 @@ template <int kMaxBits>
 __device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32_t nb,
                                                  uint32_t B, uint32_t padded,
                                                  uint8_t* dst, uint32_t lane) {
-  uint32_t t16[16];
+  if (kMaxBits <= 16 || nb <= 16) {
+    uint32_t r[8];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) t16[t] = v[t];
-  transpose16x2(t16);
+    for (int i = 0; i < 8; ++i) r[i] = (v[i] & 0xFFFFu) | (v[i + 8] << 16);
+    transpose16_packed(r);
 #pragma unroll
-  for (int b = 0; b < kMaxBits; ++b) {
-    if (b < static_cast<int>(nb)) {
-      const uint32_t bf = __brev(plane_word(t16, b));
-      uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
-      if (2u * lane < B) d[0] = static_cast<uint8_t>(bf >> 24);
-      if (2u * lane + 1u < B) d[1] = static_cast<uint8_t>(bf >> 16);
+    for (int b = 0; b < 16; ++b) {
+      if (b < static_cast<int>(nb)) {
+        const uint32_t bf = __brev(plane_word16(r, b));
+        uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
+        if (2u * lane < B) d[0] = static_cast<uint8_t>(bf >> 24);
+        if (2u * lane + 1u < B) d[1] = static_cast<uint8_t>(bf >> 16);
+      }
+    }
+  } else {
+    uint32_t t16[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) t16[t] = v[t];
+    transpose16x2(t16);
+#pragma unroll
+    for (int b = 0; b < kMaxBits; ++b) {
+      if (b < static_cast<int>(nb)) {
+        const uint32_t bf = __brev(plane_word(t16, b));
+        uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
+        if (2u * lane < B) d[0] = static_cast<uint8_t>(bf >> 24);
+        if (2u * lane + 1u < B) d[1] = static_cast<uint8_t>(bf >> 16);
+      }
     }
   }
   if (lane < padded - B * nb) dst[B * nb + lane] = 0;  // zero pad bytes
@@ -642,6 +683,10 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
   const uint32_t m = seg - 1u;
   const bool pow2 = (seg & m) == 0u;
+  // z term of the power-of-two relative position; 0 when 2*sh >= 32 (the
+  // product by seg*seg wraps to 0 in 32 bits)
+  const uint32_t zsh = (2u * sh) & 31u;
+  const uint32_t mz = 2u * sh >= 32 ? 0u : m;
   const T nmi0 = -mn[0] * inv, nmi1 = -mn[1] * inv, nmi2 = -mn[2] * inv;
   // 24-bit multiplies are exact when every factor is below 2^24; the
   // largest shifted code of an axis is the code of its maximum.
@@ -674,7 +719,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     }
     if (pow2) {
       r = (qx & m) | ((qy & m) << sh);
-      if (DIM == 3) r |= 2u * sh >= 32 ? 0u : (qz & m) << (2u * sh);
+      if (DIM == 3) r |= (qz & mz) << zsh;
     } else {
       r = (qx & m) + (qy & m) * seg;
       if (DIM == 3) r += (qz & m) * seg * seg;
@@ -777,16 +822,29 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   store_msb_planes<32>(del, db, B, D, slot, lane);
   store_msb_planes<16>(cnt, cb, B, C, slot + D, lane);
   {
-    uint32_t t16[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) t16[k] = val[k];
-    transpose16x2(t16);
     uint8_t* pr = slot + D + C;
+    if (rb <= 16) {
+      uint32_t r[8];
 #pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      if (b < static_cast<int>(rb))
-        *reinterpret_cast<uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
-            static_cast<uint16_t>(plane_word(t16, b));
+      for (int i = 0; i < 8; ++i) r[i] = (val[i] & 0xFFFFu) | (val[i + 8] << 16);
+      transpose16_packed(r);
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        if (b < static_cast<int>(rb))
+          *reinterpret_cast<uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
+              static_cast<uint16_t>(plane_word16(r, b));
+      }
+    } else {
+      uint32_t t16[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) t16[k] = val[k];
+      transpose16x2(t16);
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        if (b < static_cast<int>(rb))
+          *reinterpret_cast<uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
+              static_cast<uint16_t>(plane_word(t16, b));
+      }
     }
   }
 
@@ -1079,55 +1137,104 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   const uint32_t nvalid =
       static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
 
-  // ---- runs 16l..16l+15: deltas and counts from the MSB-first planes
+  // ---- runs 16l..16l+15: deltas and counts from the MSB-first planes.
+  // bfrev of the two plane bytes 2l, 2l+1 gives a 16-bit word with bit t =
+  // run 16l+t; the bit transpose turns plane words back into values.
   uint32_t del[16], cnt[16];
   {
-    uint32_t t16[16];
+    if (db <= 16) {
+      uint32_t r[8];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) t16[t] = 0;
+      for (int i = 0; i < 8; ++i) r[i] = 0;
 #pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      if (b < static_cast<int>(db)) {
-        const uint8_t* q = pb + (db - 1u - b) * B + 2u * lane;
-        const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
-        const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
-        const uint32_t w = __brev((b0 << 24) | (b1 << 16));
-        t16[b & 15] |= b < 16 ? w : (w << 16);
+      for (int b = 0; b < 16; ++b) {
+        if (b < static_cast<int>(db)) {
+          const uint8_t* q = pb + (db - 1u - b) * B + 2u * lane;
+          const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
+          const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
+          const uint32_t w = __brev((b0 << 24) | (b1 << 16));
+          r[b & 7] |= b < 8 ? w : (w << 16);
+        }
       }
+      transpose16_packed(r);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        del[i] = r[i] & 0xFFFFu;
+        del[i + 8] = r[i] >> 16;
+      }
+    } else {
+      uint32_t t16[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) t16[t] = 0;
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        if (b < static_cast<int>(db)) {
+          const uint8_t* q = pb + (db - 1u - b) * B + 2u * lane;
+          const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
+          const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
+          const uint32_t w = __brev((b0 << 24) | (b1 << 16));
+          t16[b & 15] |= b < 16 ? w : (w << 16);
+        }
+      }
+      transpose16x2(t16);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) del[t] = t16[t];
     }
-    transpose16x2(t16);
+    uint32_t r[8];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) del[t] = t16[t];
-#pragma unroll
-    for (int t = 0; t < 16; ++t) t16[t] = 0;
+    for (int i = 0; i < 8; ++i) r[i] = 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
       if (b < static_cast<int>(cb)) {
         const uint8_t* q = pb + D + (cb - 1u - b) * B + 2u * lane;
         const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
         const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
-        t16[b] = __brev((b0 << 24) | (b1 << 16));
+        const uint32_t w = __brev((b0 << 24) | (b1 << 16));
+        r[b & 7] |= b < 8 ? w : (w << 16);
       }
     }
-    transpose16x2(t16);
+    transpose16_packed(r);
 #pragma unroll
-    for (int t = 0; t < 16; ++t) cnt[t] = t16[t];
+    for (int i = 0; i < 8; ++i) {
+      cnt[i] = r[i] & 0xFFFFu;
+      cnt[i + 8] = r[i] >> 16;
+    }
   }
   // ---- relative positions of ranks 16l..16l+15 (LSB-first planes)
   uint32_t rel[16];
   {
-#pragma unroll
-    for (int k = 0; k < 16; ++k) rel[k] = 0;
     const uint8_t* pr = pb + D + C;
+    if (rb <= 16) {
+      uint32_t r[8];
 #pragma unroll
-    for (int b = 0; b < 32; ++b) {
-      if (b < static_cast<int>(rb)) {
-        const uint32_t w =
-            *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
-        rel[b & 15] |= b < 16 ? w : (w << 16);
+      for (int i = 0; i < 8; ++i) r[i] = 0;
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        if (b < static_cast<int>(rb)) {
+          const uint32_t w =
+              *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
+          r[b & 7] |= b < 8 ? w : (w << 16);
+        }
       }
+      transpose16_packed(r);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        rel[i] = r[i] & 0xFFFFu;
+        rel[i + 8] = r[i] >> 16;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) rel[k] = 0;
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        if (b < static_cast<int>(rb)) {
+          const uint32_t w =
+              *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
+          rel[b & 15] |= b < 16 ? w : (w << 16);
+        }
+      }
+      transpose16x2(rel);
     }
-    transpose16x2(rel);
   }
   // every plane is in registers: the stage buffer becomes the run tables
   wave_sync();
