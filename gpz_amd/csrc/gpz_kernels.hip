@@ -187,10 +187,92 @@ __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, uint32_t lane) {
   return v;
 }
 
-// Bit-matrix transpose of 16 words, two 16x16 blocks at once: on entry
-// a[i] holds item i (bits 0..31); on exit a[b] bits 0..15 hold bit b of
-// items 0..15 and bits 16..31 hold bit b+16 of items 0..15.  The operation
-// is its own inverse.  4 stages x 8 swaps of block-masked xor.
+// DPP collectives.  update_dpp moves a value between lanes inside the
+// VALU instruction that consumes it (no LDS round trip); the controls are
+// the GCN/CDNA DPP encodings.  Every reduction ends in lane 63 and is read
+// back as a uniform value.
+constexpr int kDppQuad1032 = 0xB1;      // quad_perm:[1,0,3,2]
+constexpr int kDppQuad2301 = 0x4E;      // quad_perm:[2,3,0,1]
+constexpr int kDppRowShr = 0x110;       // row_shr:n is kDppRowShr + n
+constexpr int kDppWaveShr1 = 0x138;     // wave_shr:1
+constexpr int kDppRowMirror = 0x140;
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowBcast15 = 0x142;   // lane 15 of each row -> next row
+constexpr int kDppRowBcast31 = 0x143;   // lane 31 -> rows 2 and 3
+
+// Lanes outside kRowMask, or reading from outside their row with
+// kBound set, receive `old`.
+template <int kCtrl, int kRowMask = 0xF, bool kBound = false>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(old), static_cast<int>(v), kCtrl, kRowMask, 0xF, kBound));
+}
+__device__ __forceinline__ uint32_t lane63(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+  v |= dpp<kDppQuad1032>(0u, v);
+  v |= dpp<kDppQuad2301>(0u, v);
+  v |= dpp<kDppRowHalfMirror>(0u, v);
+  v |= dpp<kDppRowMirror>(0u, v);
+  v |= dpp<kDppRowBcast15, 0xA>(0u, v);
+  v |= dpp<kDppRowBcast31, 0xC>(0u, v);
+  return lane63(v);
+}
+
+// Inclusive prefix sum over lanes (Hillis-Steele inside each row of 16,
+// then the row totals).
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
+  v += dpp<kDppRowShr + 1, 0xF, true>(0u, v);
+  v += dpp<kDppRowShr + 2, 0xF, true>(0u, v);
+  v += dpp<kDppRowShr + 4, 0xF, true>(0u, v);
+  v += dpp<kDppRowShr + 8, 0xF, true>(0u, v);
+  v += dpp<kDppRowBcast15, 0xA>(0u, v);
+  v += dpp<kDppRowBcast31, 0xC>(0u, v);
+  return v;
+}
+
+// IEEE-mode v_min/v_max on operands that are already canonical (results
+// of earlier min/max): NaN operands are quiet there and are ignored, as
+// fminf/fmaxf ignore them.
+__device__ __forceinline__ float vmin(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ float vmax(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+template <bool kMax>
+__device__ __forceinline__ float wave_ext(float v) {
+  auto step = [&](uint32_t o) {
+    v = kMax ? vmax(v, __uint_as_float(o)) : vmin(v, __uint_as_float(o));
+  };
+  uint32_t u = __float_as_uint(v);
+  step(dpp<kDppQuad1032>(u, u)); u = __float_as_uint(v);
+  step(dpp<kDppQuad2301>(u, u)); u = __float_as_uint(v);
+  step(dpp<kDppRowHalfMirror>(u, u)); u = __float_as_uint(v);
+  step(dpp<kDppRowMirror>(u, u)); u = __float_as_uint(v);
+  step(dpp<kDppRowBcast15, 0xA>(u, u)); u = __float_as_uint(v);
+  step(dpp<kDppRowBcast31, 0xC>(u, u));
+  return __uint_as_float(lane63(__float_as_uint(v)));
+}
+
+// Bit-matrix transposes.  One swap stage exchanges the m-masked bits of
+// a >> j with those of b (two shifts and two bitfield inserts).
+__device__ __forceinline__ void swap_bits(uint32_t& a, uint32_t& b, int j,
+                                          uint32_t m) {
+  const uint32_t na = ((b << j) & (m << j)) | (a & ~(m << j));
+  b = ((a >> j) & m) | (b & ~m);
+  a = na;
+}
+
+// Transpose of 16 words, two 16x16 blocks at once: on entry a[i] holds
+// item i (bits 0..31); on exit a[b] bits 0..15 hold bit b of items 0..15
+// and bits 16..31 hold bit b+16.  Its own inverse.
 __device__ __forceinline__ void transpose16x2(uint32_t (&a)[16]) {
 #pragma unroll
   for (int st = 0; st < 4; ++st) {
@@ -198,34 +280,36 @@ __device__ __forceinline__ void transpose16x2(uint32_t (&a)[16]) {
     const uint32_t m = st == 0 ? 0x00FF00FFu : st == 1 ? 0x0F0F0F0Fu
                      : st == 2 ? 0x33333333u : 0x55555555u;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (k & j) continue;
-      const uint32_t t = ((a[k] >> j) ^ a[k + j]) & m;
-      a[k + j] ^= t;
-      a[k] ^= t << j;
-    }
+    for (int k = 0; k < 16; ++k)
+      if (!(k & j)) swap_bits(a[k], a[k + j], j, m);
   }
 }
-// 16x16 bit transpose of 16-bit values held two per register: on entry
-// r[i] = item i | item i+8 << 16; on exit r[i] bits 0..15 hold bit i of
-// items 0..15 and bits 16..31 hold bit i+8.  The first stage (swap of the
-// off-diagonal 8x8 blocks) is one byte permutation per register.  Also its
-// own inverse.
-__device__ __forceinline__ void transpose16_packed(uint32_t (&r)[8]) {
-#pragma unroll
-  for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(r[i], r[i], 0x03010200u);
+
+// 16x16 transpose of 16-bit values held two per register.  pack_pair(lo,
+// hi) lays out items lo and hi (low 16 bits each) as bytes [lo.0, hi.0,
+// lo.1, hi.1]; transpose16_bytes then transposes the four 8x8 blocks
+// formed by each byte lane of r[0..7].  Starting from r[i] =
+// pack_pair(item i, item i+8) it leaves r[b] bits 0..15 = bit b of items
+// 0..15 and bits 16..31 = bit b+8; starting from r[b] = pack_pair(plane b,
+// plane b+8) it leaves r[i] = item i | item i+8 << 16.
+__device__ __forceinline__ uint32_t pack_pair(uint32_t lo, uint32_t hi) {
+  return __builtin_amdgcn_perm(hi, lo, 0x05010400u);
+}
+__device__ __forceinline__ void transpose16_bytes(uint32_t (&r)[8]) {
 #pragma unroll
   for (int st = 0; st < 3; ++st) {
     const int j = 4 >> st;
     const uint32_t m = st == 0 ? 0x0F0F0F0Fu : st == 1 ? 0x33333333u : 0x55555555u;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if (k & j) continue;
-      const uint32_t t = ((r[k] >> j) ^ r[k + j]) & m;
-      r[k + j] ^= t;
-      r[k] ^= t << j;
-    }
+    for (int k = 0; k < 8; ++k)
+      if (!(k & j)) swap_bits(r[k], r[k + j], j, m);
   }
+}
+// Same transpose from the natural layout r[i] = lo | hi << 16.
+__device__ __forceinline__ void transpose16_packed(uint32_t (&r)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(r[i], r[i], 0x03010200u);
+  transpose16_bytes(r);
 }
 __device__ __forceinline__ uint32_t plane_word16(const uint32_t (&r)[8], int b) {
   return b < 8 ? (r[b] & 0xFFFFu) : (r[b - 8] >> 16);
@@ -472,92 +556,98 @@ struct EncodeArgs {
   uint8_t* slots;   // [G * kSlot]
 };
 
+// LDS of one encode wave (bytes).  During the sort: padded keys, values by
+// position, digit counters.  After it: per-run deltas and start ranks, the
+// small-U plane stage, and the dump slot that non-start ranks write to.
+constexpr uint32_t kLdsKeyB = 0;        // u32 [1280]: 16 keys + 4 pad words per row
+constexpr uint32_t kLdsValB = 5120;     // u32 [1024]
+constexpr uint32_t kLdsCntB = 9216;     // u32 [16 digits][64 lanes]
+constexpr uint32_t kLdsBytes = 13312;
+constexpr uint32_t kRunDeltaB = 0;      // u32 [1024]
+constexpr uint32_t kRunStartB = 4096;   // u16 [1040]
+constexpr uint32_t kRunDump = 1600;     // -> bytes 6400 (delta), 7296 (start)
+constexpr uint32_t kStageB = 9216;      // <= 384 bytes
+static_assert(kRunStartB + 2 * 1040 <= 4 * kRunDump &&
+              kRunStartB + 2 * kRunDump + 2 <= kStageB &&
+              kStageB + 384 <= kLdsBytes, "LDS regions overlap");
+
 __device__ __forceinline__ uint32_t pad_word(uint32_t p) {
   return p + ((p >> 4) << 2);
 }
-__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1)
-    v |= static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m));
-  return v;
-}
 
-// Quantised code of one coordinate: sat_u32/sat_u64 of fma(x, 1/(2eb),
-// -min/(2eb)) (src/cudaWarpCompress.cu:179-184), returned modulo 2^32 after
-// the shift by log2(seg): the block id and relative position are masked to
-// 31 bits, so only the low 32 bits of every term matter
-// (src/cudaWarpCompressDouble.cu:186-193 computes them in u64).
-// v_cvt_u32_f32 truncates, clamps to [0, 2^32-1] and maps NaN to 0, the
-// same result as CUDA's cvt.rzi.u32.f32 and as sat_u32 above.
+// Quantised code of one coordinate: sat_u32 / sat_u64 of fma(x, 1/(2eb),
+// -min/(2eb)) (src/cudaWarpCompress.cu:179-184,
+// src/cudaWarpCompressDouble.cu:186-193).  v_cvt_u32_f32 truncates,
+// clamps to [0, 2^32-1] and maps NaN to 0, the same result as CUDA's
+// cvt.rzi.u32.f32 and as sat_u32 above.
 __device__ __forceinline__ uint32_t cvt_u32_sat(float f) {
   uint32_t r;
   asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
   return r;
 }
-
 template <typename T>
-__device__ __forceinline__ void quant(T x, T inv, T nmi, uint32_t sh,
-                                      uint32_t* q, uint32_t* qs) {
-  const T f = dfma(x, inv, nmi);
-  if (sizeof(T) == 4) {
-    const uint32_t v = cvt_u32_sat(static_cast<float>(f));
-    *q = v;
-    *qs = v >> sh;  // sh = ctz(seg) <= 31
-  } else {
-    const uint64_t v = sat_u64(static_cast<double>(f));
-    *q = static_cast<uint32_t>(v);
-    *qs = static_cast<uint32_t>(v >> sh);
-  }
+__device__ __forceinline__ typename Prec<T>::Q quant_q(T x, T inv, T nmi) {
+  if constexpr (sizeof(T) == 4) return cvt_u32_sat(dfma(x, inv, nmi));
+  else return sat_u64(dfma(x, inv, nmi));
 }
 
 // One pass of the stable LSD radix sort over a 4-bit digit.  Items are in
-// blocked order (lane l holds positions 16l..16l+15); each lane counts its
-// own items in a private counter column, an exclusive scan over (digit,
-// lane) gives every item its stable rank, and the items are scattered into
-// LDS and read back blocked.
+// blocked order (lane l holds positions 16l..16l+15).  Each lane counts its
+// items in its own counter column (word 64*digit + lane) with LDS atomics:
+// a column is never shared, so the atomics do not contend and their
+// return values are the lane-local ranks in item order.  An exclusive scan
+// over (digit, lane) order turns the counters into bases, the items are
+// scattered into the padded key buffer and read back blocked.  With kPair
+// the values travel along (unpadded).
 template <bool kPair>
 __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[16],
                                            uint32_t shift, uint32_t lane,
-                                           uint16_t* s_cnt, uint32_t* s_key,
-                                           uint32_t* s_val) {
-  reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(0, 0, 0, 0);
-  reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(0, 0, 0, 0);
+                                           uint8_t* lds) {
+  uint32_t* s_key = reinterpret_cast<uint32_t*>(lds + kLdsKeyB);
+  uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kLdsValB);
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kLdsCntB);
+  uint4* c4 = reinterpret_cast<uint4*>(s_cnt);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) c4[4 * lane + q] = make_uint4(0, 0, 0, 0);
   wave_sync();
-  uint32_t dig[16], lrank[16];
+  uint32_t idx[16], rnk[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    dig[k] = (key[k] >> shift) & 15u;
-    const uint32_t idx = dig[k] * 64u + lane;
-    const uint32_t o = s_cnt[idx];
-    s_cnt[idx] = static_cast<uint16_t>(o + 1u);
-    lrank[k] = o;
+    idx[k] = (__builtin_amdgcn_ubfe(key[k], shift, 4) << 6) + lane;
+    rnk[k] = __hip_atomic_fetch_add(s_cnt + idx[k], 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_WORKGROUP);
   }
   wave_sync();
   {
-    const uint4 c0 = reinterpret_cast<uint4*>(s_cnt)[2 * lane];
-    const uint4 c1 = reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1];
-    uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-    uint32_t run = 0;
+    uint32_t cv[16];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const uint32_t lo = w[i] & 0xFFFFu, hi = w[i] >> 16;
-      w[i] = run | ((run + lo) << 16);
-      run += lo + hi;
+    for (int q = 0; q < 4; ++q) {
+      const uint4 w = c4[4 * lane + q];
+      cv[4 * q] = w.x; cv[4 * q + 1] = w.y; cv[4 * q + 2] = w.z; cv[4 * q + 3] = w.w;
     }
-    const uint32_t incl = wave_incl_scan(run, lane);
-    const uint32_t add = incl - run;
-    const uint32_t add2 = add | (add << 16);
+    uint32_t tot = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) w[i] += add2;
-    reinterpret_cast<uint4*>(s_cnt)[2 * lane] = make_uint4(w[0], w[1], w[2], w[3]);
-    reinterpret_cast<uint4*>(s_cnt)[2 * lane + 1] = make_uint4(w[4], w[5], w[6], w[7]);
+    for (int i = 0; i < 16; ++i) tot += cv[i];
+    uint32_t run = wave_scan(tot) - tot;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const uint32_t t = cv[i];
+      cv[i] = run;
+      run += t;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      c4[4 * lane + q] = make_uint4(cv[4 * q], cv[4 * q + 1], cv[4 * q + 2], cv[4 * q + 3]);
   }
   wave_sync();
+  // all counter reads first, so that they are not ordered behind the
+  // scatter writes
+#pragma unroll
+  for (int k = 0; k < 16; ++k) rnk[k] += s_cnt[idx[k]];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const uint32_t wi = pad_word(s_cnt[dig[k] * 64u + lane] + lrank[k]);
-    s_key[wi] = key[k];
-    if (kPair) s_val[wi] = val[k];
+    s_key[pad_word(rnk[k])] = key[k];
+    if (kPair) s_val[rnk[k]] = val[k];
   }
   wave_sync();
 #pragma unroll
@@ -566,7 +656,7 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
     key[4 * q4 + 0] = kk.x; key[4 * q4 + 1] = kk.y;
     key[4 * q4 + 2] = kk.z; key[4 * q4 + 3] = kk.w;
     if (kPair) {
-      const uint4 vv = reinterpret_cast<const uint4*>(s_val)[5 * lane + q4];
+      const uint4 vv = reinterpret_cast<const uint4*>(s_val)[4 * lane + q4];
       val[4 * q4 + 0] = vv.x; val[4 * q4 + 1] = vv.y;
       val[4 * q4 + 2] = vv.z; val[4 * q4 + 3] = vv.w;
     }
@@ -576,100 +666,159 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
 
 // MSB-first bit planes of 16 per-lane values (runs 16l..16l+15) stored to
 // dst: plane p (bit nb-1-p) is B bytes at dst + p*B, lane l owns bytes 2l,
-// 2l+1.  The transpose gives each lane a 16-bit word per plane (bit t = run
-// 16l+t); bfrev turns it into the two MSB-first bytes.
+// 2l+1.  Feeding the transpose item t^7 in slot t makes bit s of a plane
+// word hold run 16l + (s^7), i.e. both bytes come out MSB-first.
 template <int kMaxBits>
 __device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32_t nb,
                                                  uint32_t B, uint32_t padded,
                                                  uint8_t* dst, uint32_t lane) {
+  const bool w0 = 2u * lane < B, w1 = 2u * lane + 1u < B;
   if (kMaxBits <= 16 || nb <= 16) {
     uint32_t r[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) r[i] = (v[i] & 0xFFFFu) | (v[i + 8] << 16);
-    transpose16_packed(r);
+    for (int i = 0; i < 8; ++i) r[i] = pack_pair(v[7 - i], v[15 - i]);
+    transpose16_bytes(r);
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
       if (b < static_cast<int>(nb)) {
-        const uint32_t bf = __brev(plane_word16(r, b));
+        const uint32_t w = plane_word16(r, b);
         uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
-        if (2u * lane < B) d[0] = static_cast<uint8_t>(bf >> 24);
-        if (2u * lane + 1u < B) d[1] = static_cast<uint8_t>(bf >> 16);
+        if (w0) d[0] = static_cast<uint8_t>(w);
+        if (w1) d[1] = static_cast<uint8_t>(w >> 8);
       }
     }
   } else {
     uint32_t t16[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) t16[t] = v[t];
+    for (int t = 0; t < 16; ++t) t16[t] = v[t ^ 7];
     transpose16x2(t16);
 #pragma unroll
     for (int b = 0; b < kMaxBits; ++b) {
       if (b < static_cast<int>(nb)) {
-        const uint32_t bf = __brev(plane_word(t16, b));
+        const uint32_t w = plane_word(t16, b);
         uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
-        if (2u * lane < B) d[0] = static_cast<uint8_t>(bf >> 24);
-        if (2u * lane + 1u < B) d[1] = static_cast<uint8_t>(bf >> 16);
+        if (w0) d[0] = static_cast<uint8_t>(w);
+        if (w1) d[1] = static_cast<uint8_t>(w >> 8);
       }
     }
   }
   if (lane < padded - B * nb) dst[B * nb + lane] = 0;  // zero pad bytes
 }
 
-// Coordinates of group c in blocked sort order (lane l, item k = slot
-// 32*(16*(l&1) + k) + (l>>1)); slots past N read 0.
-template <typename T, int DIM>
-__device__ __forceinline__ void load_group(const EncodeArgs& a, uint64_t c,
-                                           uint32_t lane, T (&px)[16],
-                                           T (&py)[16], T (&pz)[16]) {
-  const uint64_t base = c * kGroup;
-  const uint32_t nvalid =
-      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
-  const bool full = nvalid == kGroup;
-  const T* X = static_cast<const T*>(a.x) + base;
-  const T* Y = static_cast<const T*>(a.y) + base;
-  const T* Z = static_cast<const T*>(a.z) + base;
-  const uint32_t jbase = 512u * (lane & 1u) + (lane >> 1);
+// Stage 1-2 of one group: extrema, then block id and relative position of
+// every point (src/cudaWarpCompress.cu:120-200).  kFast: power-of-two seg
+// with 3*log2(seg) <= 30 and every shifted code, block count and block id
+// small enough that 24-bit multiplies are exact and no 31-bit mask bites.
+template <typename T, int DIM, bool kFull, bool kFast>
+__device__ __forceinline__ void quantise(const T (&px)[16], const T (&py)[16],
+                                         const T (&pz)[16], T inv, const T (&nmi)[3],
+                                         uint32_t seg, uint32_t nbx32, uint32_t nbxy32,
+                                         uint32_t jbase, uint32_t nvalid,
+                                         uint32_t (&key)[16], uint32_t (&val)[16],
+                                         uint32_t* bor_out, uint32_t* ror_out) {
+  const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
+  const uint32_t m = seg - 1u;
+  const bool pow2 = (seg & m) == 0u;
+  // z term of the power-of-two relative position; 0 when 2*sh >= 32 (the
+  // product by seg*seg wraps to 0 in 32 bits)
+  const uint32_t zsh = (2u * sh) & 31u;
+  const uint32_t mz = 2u * sh >= 32 ? 0u : m;
+  uint32_t bor = 0, ror = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const uint32_t j = jbase + 32u * k;
-    const bool v = full || j < nvalid;
-    px[k] = v ? X[j] : T(0);
-    py[k] = v ? Y[j] : T(0);
-    pz[k] = (DIM == 3 && v) ? Z[j] : T(0);
+    const auto qx = quant_q<T>(px[k], inv, nmi[0]);
+    const auto qy = quant_q<T>(py[k], inv, nmi[1]);
+    const auto qz = DIM == 3 ? quant_q<T>(pz[k], inv, nmi[2]) : decltype(qx)(0);
+    const uint32_t sx = static_cast<uint32_t>(qx >> sh);
+    const uint32_t sy = static_cast<uint32_t>(qy >> sh);
+    const uint32_t sz = static_cast<uint32_t>(qz >> sh);
+    const uint32_t lx = static_cast<uint32_t>(qx), ly = static_cast<uint32_t>(qy),
+                   lz = static_cast<uint32_t>(qz);
+    uint32_t b, r;
+    if (kFast) {
+      b = sx + __umul24(sy, nbx32);
+      if (DIM == 3) b += __umul24(sz, nbxy32);
+      r = (lx & m) | ((ly & m) << sh);
+      if (DIM == 3) r |= (lz & m) << (2u * sh);
+    } else {
+      b = sx + sy * nbx32;
+      if (DIM == 3) b += sz * nbxy32;
+      if (pow2) {
+        r = (lx & m) | ((ly & m) << sh);
+        if (DIM == 3) r |= (lz & mz) << zsh;
+      } else {
+        r = (lx & m) + (ly & m) * seg;
+        if (DIM == 3) r += (lz & m) * seg * seg;
+      }
+      b &= 0x7FFFFFFFu;
+      r &= 0x7FFFFFFFu;
+    }
+    if (kFull) {
+      key[k] = b;
+      val[k] = r;
+      bor |= b;
+      ror |= r;
+    } else {
+      const bool v = jbase + 32u * k < nvalid;
+      key[k] = v ? b : 0xFFFFFFFFu;
+      val[k] = v ? r : 0xFFFFFFFFu;
+      bor |= v ? b : 1u;
+      ror |= v ? r : 1u;
+    }
   }
+  *bor_out = bor;
+  *ror_out = ror;
 }
 
-template <typename T, int DIM>
+template <typename T, int DIM, bool kFull>
 __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
-                                             uint32_t lane, const T (&px)[16],
-                                             const T (&py)[16],
-                                             const T (&pz)[16], uint32_t* s_a,
-                                             uint16_t* s_cnt) {
+                                             uint32_t lane, uint8_t* lds) {
   using Q = typename Prec<T>::Q;
   const uint32_t seg = a.info->seg_size;
   const T eb = static_cast<T>(a.eb);
   const uint64_t base = c * kGroup;
-  const uint32_t nvalid =
-      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
-  const bool full = nvalid == kGroup;
+  const uint32_t nvalid = kFull ? kGroup : static_cast<uint32_t>(a.n - base);
   const bool last = (c == a.G - 1);
   const uint32_t jbase = 512u * (lane & 1u) + (lane >> 1);
 
+  // ---- stage 0: coordinates in blocked sort order (lane l, item k = slot
+  // 32*(16*(l&1) + k) + (l>>1)).  In the partial last group, slots past N
+  // re-read the group's last point: the extrema do not change and their
+  // keys are replaced below.
+  T px[16], py[16], pz[16];
+  {
+    const T* X = static_cast<const T*>(a.x) + base;
+    const T* Y = static_cast<const T*>(a.y) + base;
+    const T* Z = static_cast<const T*>(a.z) + base;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      uint32_t j = jbase + 32u * k;
+      if (!kFull) j = j < nvalid ? j : nvalid - 1u;
+      px[k] = X[j];
+      py[k] = Y[j];
+      pz[k] = DIM == 3 ? Z[j] : T(0);
+    }
+  }
+
   // ---- stage 1: group extrema
-  T mn[3] = {Prec<T>::lo(), Prec<T>::lo(), Prec<T>::lo()};
-  T mx[3] = {-Prec<T>::lo(), -Prec<T>::lo(), -Prec<T>::lo()};
+  T mn[3], mx[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) { mn[d] = Prec<T>::lo(); mx[d] = -Prec<T>::lo(); }
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const uint32_t j = jbase + 32u * k;
-    if (full || j < nvalid) {
-      mn[0] = dmin(mn[0], px[k]); mx[0] = dmax(mx[0], px[k]);
-      mn[1] = dmin(mn[1], py[k]); mx[1] = dmax(mx[1], py[k]);
-      if (DIM == 3) { mn[2] = dmin(mn[2], pz[k]); mx[2] = dmax(mx[2], pz[k]); }
-    }
+    mn[0] = dmin(mn[0], px[k]); mx[0] = dmax(mx[0], px[k]);
+    mn[1] = dmin(mn[1], py[k]); mx[1] = dmax(mx[1], py[k]);
+    if (DIM == 3) { mn[2] = dmin(mn[2], pz[k]); mx[2] = dmax(mx[2], pz[k]); }
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    mn[d] = wave_min(mn[d]);
-    mx[d] = wave_max(mx[d]);
+    if constexpr (sizeof(T) == 4) {
+      mn[d] = wave_ext<false>(mn[d]);
+      mx[d] = wave_ext<true>(mx[d]);
+    } else {
+      mn[d] = wave_min(mn[d]);
+      mx[d] = wave_max(mx[d]);
+    }
   }
   if (DIM == 2) { mn[2] = T(0); mx[2] = T(0); }
 
@@ -680,58 +829,29 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   block_counts<T, DIM>(mn[0], mx[0], mn[1], mx[1], seg2eb, &nbx, &nby);
   const uint32_t nbx32 = static_cast<uint32_t>(nbx);
   const uint32_t nbxy32 = static_cast<uint32_t>(nbx) * static_cast<uint32_t>(nby);
+  const T nmi[3] = {-mn[0] * inv, -mn[1] * inv, -mn[2] * inv};
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
-  const uint32_t m = seg - 1u;
-  const bool pow2 = (seg & m) == 0u;
-  // z term of the power-of-two relative position; 0 when 2*sh >= 32 (the
-  // product by seg*seg wraps to 0 in 32 bits)
-  const uint32_t zsh = (2u * sh) & 31u;
-  const uint32_t mz = 2u * sh >= 32 ? 0u : m;
-  const T nmi0 = -mn[0] * inv, nmi1 = -mn[1] * inv, nmi2 = -mn[2] * inv;
-  // 24-bit multiplies are exact when every factor is below 2^24; the
-  // largest shifted code of an axis is the code of its maximum.
-  uint32_t qtop = 0, dummy;
-  {
-    uint32_t qs0, qs1, qs2 = 0;
-    quant<T>(mx[0], inv, nmi0, sh, &dummy, &qs0);
-    quant<T>(mx[1], inv, nmi1, sh, &dummy, &qs1);
-    if (DIM == 3) quant<T>(mx[2], inv, nmi2, sh, &dummy, &qs2);
-    qtop = qs0 | qs1 | qs2 | nbx32 | (DIM == 3 ? nbxy32 : 0u);
-    if (sizeof(T) == 8 && (static_cast<uint64_t>(nbx) >> 32)) qtop = ~0u;
+  // The fast path needs every point's shifted code below the code of the
+  // axis maximum, which holds while x -> fma(x, inv, nmi) is monotone
+  // (inv finite and positive).
+  bool fast = ((seg & (seg - 1u)) == 0u) && sh <= (DIM == 3 ? 10u : 15u) &&
+              inv > T(0) && inv <= Prec<T>::lo();
+  if (fast) {
+    const uint64_t s0 = static_cast<uint64_t>(quant_q<T>(mx[0], inv, nmi[0]) >> sh);
+    const uint64_t s1 = static_cast<uint64_t>(quant_q<T>(mx[1], inv, nmi[1]) >> sh);
+    const uint64_t s2 = DIM == 3 ? static_cast<uint64_t>(quant_q<T>(mx[2], inv, nmi[2]) >> sh) : 0;
+    const uint64_t bx = static_cast<uint64_t>(nbx);
+    const uint64_t bxy = DIM == 3 ? bx * static_cast<uint64_t>(nby) : 0;
+    fast = ((s0 | s1 | s2 | bx | bxy) >> 24) == 0 &&
+           s0 + s1 * bx + s2 * bxy <= 0x7FFFFFFFull;
   }
-  const bool fast24 = qtop < (1u << 24);
-  uint32_t key[16], val[16];
-  uint32_t bor = 0, ror = 0;
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const uint32_t j = jbase + 32u * k;
-    uint32_t qx, qy, qz = 0, sx, sy, sz = 0;
-    quant<T>(px[k], inv, nmi0, sh, &qx, &sx);
-    quant<T>(py[k], inv, nmi1, sh, &qy, &sy);
-    if (DIM == 3) quant<T>(pz[k], inv, nmi2, sh, &qz, &sz);
-    uint32_t b, r;
-    if (fast24) {
-      b = sx + __umul24(sy, nbx32);
-      if (DIM == 3) b += __umul24(sz, nbxy32);
-    } else {
-      b = sx + sy * nbx32;
-      if (DIM == 3) b += sz * nbxy32;
-    }
-    if (pow2) {
-      r = (qx & m) | ((qy & m) << sh);
-      if (DIM == 3) r |= (qz & mz) << zsh;
-    } else {
-      r = (qx & m) + (qy & m) * seg;
-      if (DIM == 3) r += (qz & m) * seg * seg;
-    }
-    b &= 0x7FFFFFFFu;
-    r &= 0x7FFFFFFFu;
-    const bool v = full || j < nvalid;
-    key[k] = v ? b : 0xFFFFFFFFu;
-    val[k] = v ? r : 0xFFFFFFFFu;
-    bor |= v ? b : 1u;
-    ror |= v ? r : 1u;
-  }
+  uint32_t key[16], val[16], bor, ror;
+  if (fast)
+    quantise<T, DIM, kFull, true>(px, py, pz, inv, nmi, seg, nbx32, nbxy32, jbase,
+                                  nvalid, key, val, &bor, &ror);
+  else
+    quantise<T, DIM, kFull, false>(px, py, pz, inv, nmi, seg, nbx32, nbxy32, jbase,
+                                   nvalid, key, val, &bor, &ror);
   const uint32_t bidbits = eff_bits(wave_or(bor));
   const uint32_t rb = eff_bits(wave_or(ror));
 
@@ -739,9 +859,8 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   // blocked position order.  With <= 22 key bits the position rides in the
   // low 10 bits of one word and the values stay put, indexed by position.
   const uint32_t sortbits = bidbits + (last ? 1u : 0u);
-  uint32_t* s_key = s_a;
-  uint32_t* s_val = s_a + kSortWords;
   if (sortbits <= 22) {
+    uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kLdsValB);
 #pragma unroll
     for (int q4 = 0; q4 < 4; ++q4)
       reinterpret_cast<uint4*>(s_val)[4 * lane + q4] =
@@ -749,8 +868,8 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 #pragma unroll
     for (int k = 0; k < 16; ++k) key[k] = (key[k] << 10) | (16u * lane + k);
     for (uint32_t shift = 10; shift < 10 + sortbits; shift += 4)
-      radix_pass<false>(key, val, shift, lane, s_cnt, s_key, s_val);
-    if (sortbits == 0) wave_sync();
+      radix_pass<false>(key, val, shift, lane, lds);
+    wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       val[k] = s_val[key[k] & 1023u];
@@ -759,75 +878,139 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     // invalid ranks keep all-ones block ids (never compared below)
   } else {
     for (uint32_t shift = 0; shift < sortbits; shift += 4)
-      radix_pass<true>(key, val, shift, lane, s_cnt, s_key, s_val);
+      radix_pass<true>(key, val, shift, lane, lds);
   }
   wave_sync();
 
-  // ---- stage 4a: runs of equal block id (lane l holds ranks 16l..16l+15)
-  uint16_t* s_rstart = reinterpret_cast<uint16_t*>(s_a);          // [1040]
-  uint32_t* s_rdelta = s_a + 520;                                 // [1024]
-  const uint32_t prev0 =
-      static_cast<uint32_t>(__shfl_up(static_cast<int>(key[15]), 1));
-  uint32_t fl = 0;
+  // ---- stage 4a: runs of equal block id (lane l holds ranks 16l..16l+15).
+  // Run u's delta and start rank go to u32 / u16 tables; ranks that start
+  // no run write to a dump slot instead of branching.
+  uint32_t* s_rdelta = reinterpret_cast<uint32_t*>(lds + kRunDeltaB);
+  uint16_t* s_rstart = reinterpret_cast<uint16_t*>(lds + kRunStartB);
+  const uint32_t up = dpp<kDppWaveShr1>(0u, key[15]);
+  const uint32_t prev0 = lane == 0 ? key[0] : up;
+  bool f[16];
+  uint32_t nf = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const uint32_t r = 16u * lane + k;
-    const uint32_t prev = k ? key[k - 1] : prev0;
-    const bool f = (r < nvalid) && (r == 0 || key[k] != prev);
-    fl |= static_cast<uint32_t>(f) << k;
+    f[k] = key[k] != (k ? key[k - 1] : prev0);
+    if (k == 0) f[0] = f[0] || lane == 0;
+    if (!kFull) f[k] = f[k] && (16u * lane + k < nvalid);
+    nf += f[k] ? 1u : 0u;
   }
-  const uint32_t nf = __popc(fl);
-  const uint32_t fincl = wave_incl_scan(nf, lane);
-  const uint32_t U = static_cast<uint32_t>(__shfl(static_cast<int>(fincl), 63));
+  const uint32_t fincl = wave_scan(nf);
+  const uint32_t U = lane63(fincl);
+  const bool small = U <= 64;
   {
-    uint32_t idx = fincl - nf;
+    // deltas past U read 0 and starts past U read nvalid, so that counts
+    // past U come out 0
+    const uint32_t nv2 = nvalid | (nvalid << 16);
+    uint4* d4 = reinterpret_cast<uint4*>(s_rdelta);
+    uint4* s4 = reinterpret_cast<uint4*>(s_rstart);
+    const uint4 z4 = make_uint4(0, 0, 0, 0), n4 = make_uint4(nv2, nv2, nv2, nv2);
+    if (small) {
+      if (lane < 16) d4[lane] = z4;
+      if (lane < 9) s4[lane] = n4;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) d4[64 * q + lane] = z4;
+      s4[lane] = n4;
+      s4[64 + lane] = n4;
+      if (lane < 2) s4[128 + lane] = n4;
+    }
+    uint32_t ridx = fincl - nf;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      if ((fl >> k) & 1u) {
-        const uint32_t r = 16u * lane + k;
-        const uint32_t prev = k ? key[k - 1] : prev0;
-        s_rstart[idx] = static_cast<uint16_t>(r);
-        s_rdelta[idx] = r == 0 ? 0u : key[k] - prev;
-        ++idx;
+      const uint32_t w = f[k] ? ridx : kRunDump;
+      s_rdelta[w] = key[k] - (k ? key[k - 1] : prev0);  // run 0: prev0 = key[0]
+      s_rstart[w] = static_cast<uint16_t>(16u * lane + k);
+      ridx += f[k] ? 1u : 0u;
+    }
+  }
+  const uint32_t outlier = static_cast<uint32_t>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(key[0])));
+  wave_sync();
+
+  // ---- stage 4b: delta and count planes, MSB-first over run index
+  uint8_t* slot = a.slots + c * static_cast<uint64_t>(kSlot);
+  uint32_t db, cb, D, C, R;
+  if (small) {
+    // One run per lane, lane l holding run l^7: a ballot of bit b is then
+    // plane b with every byte already MSB-first.  Lane j < B writes byte j
+    // of each plane into the stage, which is copied out as dwords.
+    const uint32_t u = lane ^ 7u;
+    const uint32_t d = s_rdelta[u];
+    const uint32_t cn = static_cast<uint32_t>(s_rstart[u + 1]) - s_rstart[u];
+    db = eff_bits(wave_or(d));
+    cb = eff_bits(wave_or(cn));
+    section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
+    const uint32_t B = (U + 7u) >> 3;
+    uint8_t* st = lds + kStageB;
+    uint32_t* st32 = reinterpret_cast<uint32_t*>(st);
+    const uint32_t nw = (D + C) >> 2;
+    if (lane < nw) st32[lane] = 0;
+    if (lane + 64 < nw) st32[lane + 64] = 0;
+    const uint32_t sb = 8u * lane;
+    uint32_t addr = (db - 1u) * B + lane;  // plane db-1-b, byte `lane`
+#pragma unroll
+    for (int b = 0; b < 32; ++b) {
+      if (b < static_cast<int>(db)) {
+        const uint64_t pm = __builtin_amdgcn_ballot_w64(((d >> b) & 1u) != 0u);
+        if (lane < B) st[addr] = static_cast<uint8_t>(pm >> sb);
+        addr -= B;
       }
     }
-    if (lane == 0) s_rstart[U] = static_cast<uint16_t>(nvalid);
-  }
-  const uint32_t outlier =
-      static_cast<uint32_t>(__shfl(static_cast<int>(key[0]), 0));
-  wave_sync();
-  uint32_t del[16], cnt[16];
-  uint32_t dor = 0, cor = 0;
+    addr = D + (cb - 1u) * B + lane;
 #pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const uint32_t i = 16u * lane + t;
-    del[t] = 0;
-    cnt[t] = 0;
-    if (i < U) {
-      del[t] = s_rdelta[i];
-      cnt[t] = static_cast<uint32_t>(s_rstart[i + 1]) - s_rstart[i];
+    for (int b = 0; b < 16; ++b) {
+      if (b < static_cast<int>(cb)) {
+        const uint64_t pm = __builtin_amdgcn_ballot_w64(((cn >> b) & 1u) != 0u);
+        if (lane < B) st[addr] = static_cast<uint8_t>(pm >> sb);
+        addr -= B;
+      }
     }
-    dor |= del[t];
-    cor |= cnt[t];
+    wave_sync();
+    uint32_t* sl32 = reinterpret_cast<uint32_t*>(slot);
+    if (lane < nw) sl32[lane] = st32[lane];
+    if (lane + 64 < nw) sl32[lane + 64] = st32[lane + 64];
+  } else {
+    uint32_t del[16], cnt[16];
+    {
+      const uint4* d4 = reinterpret_cast<const uint4*>(s_rdelta) + 4 * lane;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint4 w = d4[q];
+        del[4 * q] = w.x; del[4 * q + 1] = w.y; del[4 * q + 2] = w.z; del[4 * q + 3] = w.w;
+      }
+      const uint4* s4 = reinterpret_cast<const uint4*>(s_rstart) + 2 * lane;
+      const uint4 a0 = s4[0], a1 = s4[1];
+      const uint32_t sw[9] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
+                              static_cast<uint32_t>(s_rstart[16 * lane + 16])};
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        cnt[2 * i] = (sw[i] >> 16) - (sw[i] & 0xFFFFu);
+        cnt[2 * i + 1] = (sw[i + 1] & 0xFFFFu) - (sw[i] >> 16);
+      }
+    }
+    uint32_t dor = 0, cor = 0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) { dor |= del[t]; cor |= cnt[t]; }
+    db = eff_bits(wave_or(dor));
+    cb = eff_bits(wave_or(cor));
+    section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
+    const uint32_t B = (U + 7u) >> 3;
+    store_msb_planes<32>(del, db, B, D, slot, lane);
+    store_msb_planes<16>(cnt, cb, B, C, slot + D, lane);
   }
-  const uint32_t db = eff_bits(wave_or(dor));
-  const uint32_t cb = eff_bits(wave_or(cor));
-  uint32_t D, C, R;
-  section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
-  const uint32_t loc = D + C + R;
-  const uint32_t B = (U + 7u) >> 3;
 
-  // ---- stage 4b: bit planes straight into the group's slot
-  uint8_t* slot = a.slots + c * static_cast<uint64_t>(kSlot);
-  store_msb_planes<32>(del, db, B, D, slot, lane);
-  store_msb_planes<16>(cnt, cb, B, C, slot + D, lane);
+  // ---- stage 4c: relative-position planes, LSB-first over sorted rank
   {
     uint8_t* pr = slot + D + C;
     if (rb <= 16) {
       uint32_t r[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) r[i] = (val[i] & 0xFFFFu) | (val[i + 8] << 16);
-      transpose16_packed(r);
+      for (int i = 0; i < 8; ++i) r[i] = pack_pair(val[i], val[i + 8]);
+      transpose16_bytes(r);
 #pragma unroll
       for (int b = 0; b < 16; ++b) {
         if (b < static_cast<int>(rb))
@@ -882,7 +1065,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     if (lane < M / 4)
       reinterpret_cast<uint32_t*>(a.meta + c * M)[lane] = word;
   }
-  if (lane == 0) a.loc[c] = loc;
+  if (lane == 0) a.loc[c] = D + C + R;
 }
 
 // One wave per group.  (A variant that prefetched the next group's
@@ -890,17 +1073,14 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 // halves occupancy.)
 template <typename T, int DIM>
 __global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
-  // s_a: sort keys (first 5120 B) and values or relative positions indexed
-  // by blocked position (second 5120 B); after the sort it holds the run
-  // arrays.  s_cnt: radix digit counters.
-  __shared__ __align__(16) uint32_t s_a[2 * kSortWords];
-  __shared__ __align__(16) uint16_t s_cnt[16 * 64];
+  __shared__ __align__(16) uint8_t lds[kLdsBytes];
   const uint32_t lane = threadIdx.x;
   if (a.info->status != GPZ_OK) return;
   const uint64_t c = blockIdx.x;
-  T px[16], py[16], pz[16];
-  load_group<T, DIM>(a, c, lane, px, py, pz);
-  encode_group<T, DIM>(a, c, lane, px, py, pz, s_a, s_cnt);
+  if ((c + 1) * kGroup <= a.n)
+    encode_group<T, DIM, true>(a, c, lane, lds);
+  else
+    encode_group<T, DIM, false>(a, c, lane, lds);
 }
 
 // ------------------------------------------------- offsets scan + header
