@@ -593,22 +593,41 @@ __device__ __forceinline__ typename Prec<T>::Q quant_q(T x, T inv, T nmi) {
 
 // One pass of the stable LSD radix sort over a 4-bit digit.  Items are in
 // blocked order (lane l holds positions 16l..16l+15).  Each lane counts its
-// items in its own counter column (word 64*digit + lane) with LDS atomics:
-// a column is never shared, so the atomics do not contend and their
-// return values are the lane-local ranks in item order.  An exclusive scan
-// over (digit, lane) order turns the counters into bases, the items are
-// scattered into the padded key buffer and read back blocked.  With kPair
-// the values travel along (unpadded).
+// items in its own counter column (word 64*digit + lane, i.e. bank = lane)
+// with LDS atomics: a column is never shared, so the atomics neither
+// contend nor conflict and their return values are the lane-local ranks in
+// item order.  An exclusive scan over (digit, lane) order turns the
+// counters into bases: lane L scans the 16 counters [16L, 16L+16), held as
+// four 16-byte chunks that it visits in the order rot[q] = (q + a) & 3,
+// a = (L >> 2) & 3, so that the 16 lanes served in one LDS cycle hit 16
+// different bank groups.  The items are then scattered into the padded key
+// buffer and read back blocked.  With kPair the values travel along
+// (unpadded).
+struct ScanLane {
+  uint32_t chunk[4];  // uint4 index of the chunk visited at step q
+  bool wrap1, wrap2, wrap3;  // step q holds a chunk before rot[0] (a + q >= 4)
+  bool a1, a2, a3;
+};
+__device__ __forceinline__ ScanLane scan_lane(uint32_t lane) {
+  ScanLane s;
+  const uint32_t a = (lane >> 2) & 3u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) s.chunk[q] = 4u * lane + ((q + a) & 3u);
+  s.wrap1 = a >= 3; s.wrap2 = a >= 2; s.wrap3 = a >= 1;
+  s.a1 = a == 1; s.a2 = a == 2; s.a3 = a == 3;
+  return s;
+}
+
 template <bool kPair>
 __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[16],
                                            uint32_t shift, uint32_t lane,
-                                           uint8_t* lds) {
+                                           const ScanLane& sl, uint8_t* lds) {
   uint32_t* s_key = reinterpret_cast<uint32_t*>(lds + kLdsKeyB);
   uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kLdsValB);
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kLdsCntB);
   uint4* c4 = reinterpret_cast<uint4*>(s_cnt);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) c4[4 * lane + q] = make_uint4(0, 0, 0, 0);
+  for (int q = 0; q < 4; ++q) c4[sl.chunk[q]] = make_uint4(0, 0, 0, 0);
   wave_sync();
   uint32_t idx[16], rnk[16];
 #pragma unroll
@@ -619,25 +638,31 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
   }
   wave_sync();
   {
-    uint32_t cv[16];
+    uint32_t cv[16], S[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint4 w = c4[4 * lane + q];
+      const uint4 w = c4[sl.chunk[q]];
       cv[4 * q] = w.x; cv[4 * q + 1] = w.y; cv[4 * q + 2] = w.z; cv[4 * q + 3] = w.w;
+      S[q] = w.x + w.y + w.z + w.w;
     }
-    uint32_t tot = 0;
+    // P[q]: sum of the chunks visited before step q; the chunk at step q
+    // starts at ex + P[q] - P[4-a] (+ T when it lies after chunk a).
+    const uint32_t P1 = S[0], P2 = S[0] + S[1], P3 = P2 + S[2], T = P3 + S[3];
+    const uint32_t Pa = sl.a1 ? P3 : sl.a2 ? P2 : sl.a3 ? P1 : T;
+    const uint32_t X = wave_scan(T) - T - Pa;
+    const uint32_t Y = X + T;
+    uint32_t run[4] = {Y, (sl.wrap1 ? X : Y) + P1, (sl.wrap2 ? X : Y) + P2,
+                       (sl.wrap3 ? X : Y) + P3};
 #pragma unroll
-    for (int i = 0; i < 16; ++i) tot += cv[i];
-    uint32_t run = wave_scan(tot) - tot;
+    for (int q = 0; q < 4; ++q) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      const uint32_t t = cv[i];
-      cv[i] = run;
-      run += t;
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t t = cv[4 * q + e];
+        cv[4 * q + e] = run[q];
+        run[q] += t;
+      }
+      c4[sl.chunk[q]] = make_uint4(cv[4 * q], cv[4 * q + 1], cv[4 * q + 2], cv[4 * q + 3]);
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      c4[4 * lane + q] = make_uint4(cv[4 * q], cv[4 * q + 1], cv[4 * q + 2], cv[4 * q + 3]);
   }
   wave_sync();
   // all counter reads first, so that they are not ordered behind the
@@ -859,6 +884,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   // blocked position order.  With <= 22 key bits the position rides in the
   // low 10 bits of one word and the values stay put, indexed by position.
   const uint32_t sortbits = bidbits + (last ? 1u : 0u);
+  const ScanLane sl = scan_lane(lane);
   if (sortbits <= 22) {
     uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kLdsValB);
 #pragma unroll
@@ -868,7 +894,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 #pragma unroll
     for (int k = 0; k < 16; ++k) key[k] = (key[k] << 10) | (16u * lane + k);
     for (uint32_t shift = 10; shift < 10 + sortbits; shift += 4)
-      radix_pass<false>(key, val, shift, lane, lds);
+      radix_pass<false>(key, val, shift, lane, sl, lds);
     wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -878,7 +904,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     // invalid ranks keep all-ones block ids (never compared below)
   } else {
     for (uint32_t shift = 0; shift < sortbits; shift += 4)
-      radix_pass<true>(key, val, shift, lane, lds);
+      radix_pass<true>(key, val, shift, lane, sl, lds);
   }
   wave_sync();
 
