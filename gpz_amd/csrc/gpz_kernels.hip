@@ -74,7 +74,6 @@ struct LaunchTimer {
 
 constexpr uint32_t kGroup = 1024;
 constexpr uint32_t kSlot = 9344;       // max payload bytes of one group
-constexpr uint32_t kSortWords = 1280;  // 1024 words + 4 pad words per 16
 constexpr uint32_t kSampleLen = 256;   // src/cudaWarpCompress.cu:1425-1426
 constexpr uint32_t kSampleBlocks = 1024;
 constexpr uint32_t kMinMaxBlocks = 1024;
@@ -556,23 +555,31 @@ struct EncodeArgs {
   uint8_t* slots;   // [G * kSlot]
 };
 
-// LDS of one encode wave (bytes).  During the sort: padded keys, values by
-// position, digit counters.  After it: per-run deltas and start ranks, the
-// small-U plane stage, and the dump slot that non-start ranks write to.
-constexpr uint32_t kLdsKeyB = 0;        // u32 [1280]: 16 keys + 4 pad words per row
-constexpr uint32_t kLdsValB = 5120;     // u32 [1024]
-constexpr uint32_t kLdsCntB = 9216;     // u32 [16 digits][64 lanes]
-constexpr uint32_t kLdsBytes = 13312;
+// LDS of one encode wave (bytes).  During the sort: padded keys and the
+// digit counters (the pair sort parks its values in the counter region
+// between the counter reads and the next pass).  After it: values by
+// position for the gather, then per-run deltas and start ranks, the
+// per-lane dump slots that non-start ranks write to, and the small-U plane
+// stage.  8448 bytes let 16 waves share a CU.
+constexpr uint32_t kLdsKeyB = 0;        // u32 [1088]: 16 keys + 1 pad word per row
+constexpr uint32_t kLdsCntB = 4352;     // u32 [16 digits][64 lanes]
+constexpr uint32_t kLdsBytes = 8448;
+constexpr uint32_t kGatherB = 0;        // u32 [1024] values by position
 constexpr uint32_t kRunDeltaB = 0;      // u32 [1024]
 constexpr uint32_t kRunStartB = 4096;   // u16 [1040]
-constexpr uint32_t kRunDump = 1600;     // -> bytes 6400 (delta), 7296 (start)
-constexpr uint32_t kStageB = 9216;      // <= 384 bytes
+constexpr uint32_t kRunDump = 1600;     // + lane -> bytes 6400.. (delta), 7296.. (start)
+constexpr uint32_t kStageB = 7424;      // <= 384 bytes
 static_assert(kRunStartB + 2 * 1040 <= 4 * kRunDump &&
-              kRunStartB + 2 * kRunDump + 2 <= kStageB &&
-              kStageB + 384 <= kLdsBytes, "LDS regions overlap");
+              4 * (kRunDump + 64) <= kRunStartB + 2 * kRunDump &&
+              kRunStartB + 2 * (kRunDump + 64) <= kStageB &&
+              kStageB + 384 <= kLdsBytes && kLdsCntB >= 4 * 1088,
+              "LDS regions overlap");
 
+// Sort key buffer: position p at word p + p/16.  Rows of 17 words keep
+// both the blocked read-back (lane l reads 17l + k) and an identity
+// scatter free of bank conflicts on 64 banks.
 __device__ __forceinline__ uint32_t pad_word(uint32_t p) {
-  return p + ((p >> 4) << 2);
+  return p + (p >> 4);
 }
 
 // Quantised code of one coordinate: sat_u32 / sat_u64 of fma(x, 1/(2eb),
@@ -623,8 +630,8 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
                                            uint32_t shift, uint32_t lane,
                                            const ScanLane& sl, uint8_t* lds) {
   uint32_t* s_key = reinterpret_cast<uint32_t*>(lds + kLdsKeyB);
-  uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kLdsValB);
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kLdsCntB);
+  uint32_t* s_val = s_cnt;  // free once every counter has been read
   uint4* c4 = reinterpret_cast<uint4*>(s_cnt);
 #pragma unroll
   for (int q = 0; q < 4; ++q) c4[sl.chunk[q]] = make_uint4(0, 0, 0, 0);
@@ -665,10 +672,11 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
     }
   }
   wave_sync();
-  // all counter reads first, so that they are not ordered behind the
-  // scatter writes
+  // all counter reads first: they must precede the scatter writes (the
+  // values overwrite the counters) and are then not ordered behind them
 #pragma unroll
   for (int k = 0; k < 16; ++k) rnk[k] += s_cnt[idx[k]];
+  if (kPair) wave_sync();
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     s_key[pad_word(rnk[k])] = key[k];
@@ -676,10 +684,9 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
   }
   wave_sync();
 #pragma unroll
+  for (int k = 0; k < 16; ++k) key[k] = s_key[17 * lane + k];
+#pragma unroll
   for (int q4 = 0; q4 < 4; ++q4) {
-    const uint4 kk = reinterpret_cast<const uint4*>(s_key)[5 * lane + q4];
-    key[4 * q4 + 0] = kk.x; key[4 * q4 + 1] = kk.y;
-    key[4 * q4 + 2] = kk.z; key[4 * q4 + 3] = kk.w;
     if (kPair) {
       const uint4 vv = reinterpret_cast<const uint4*>(s_val)[4 * lane + q4];
       val[4 * q4 + 0] = vv.x; val[4 * q4 + 1] = vv.y;
@@ -886,15 +893,16 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   const uint32_t sortbits = bidbits + (last ? 1u : 0u);
   const ScanLane sl = scan_lane(lane);
   if (sortbits <= 22) {
-    uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kLdsValB);
-#pragma unroll
-    for (int q4 = 0; q4 < 4; ++q4)
-      reinterpret_cast<uint4*>(s_val)[4 * lane + q4] =
-          make_uint4(val[4 * q4], val[4 * q4 + 1], val[4 * q4 + 2], val[4 * q4 + 3]);
 #pragma unroll
     for (int k = 0; k < 16; ++k) key[k] = (key[k] << 10) | (16u * lane + k);
     for (uint32_t shift = 10; shift < 10 + sortbits; shift += 4)
       radix_pass<false>(key, val, shift, lane, sl, lds);
+    wave_sync();
+    uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kGatherB);
+#pragma unroll
+    for (int q4 = 0; q4 < 4; ++q4)
+      reinterpret_cast<uint4*>(s_val)[4 * lane + q4] =
+          make_uint4(val[4 * q4], val[4 * q4 + 1], val[4 * q4 + 2], val[4 * q4 + 3]);
     wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -910,7 +918,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 
   // ---- stage 4a: runs of equal block id (lane l holds ranks 16l..16l+15).
   // Run u's delta and start rank go to u32 / u16 tables; ranks that start
-  // no run write to a dump slot instead of branching.
+  // no run write to a per-lane dump slot instead of branching.
   uint32_t* s_rdelta = reinterpret_cast<uint32_t*>(lds + kRunDeltaB);
   uint16_t* s_rstart = reinterpret_cast<uint16_t*>(lds + kRunStartB);
   const uint32_t up = dpp<kDppWaveShr1>(0u, key[15]);
@@ -947,7 +955,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     uint32_t ridx = fincl - nf;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const uint32_t w = f[k] ? ridx : kRunDump;
+      const uint32_t w = f[k] ? ridx : kRunDump + lane;
       s_rdelta[w] = key[k] - (k ? key[k - 1] : prev0);  // run 0: prev0 = key[0]
       s_rstart[w] = static_cast<uint16_t>(16u * lane + k);
       ridx += f[k] ? 1u : 0u;
@@ -1098,7 +1106,8 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 // coordinates into registers measured slower: the doubled register set
 // halves occupancy.)
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) void encode_kernel(EncodeArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void encode_kernel(EncodeArgs a) {
   __shared__ __align__(16) uint8_t lds[kLdsBytes];
   const uint32_t lane = threadIdx.x;
   if (a.info->status != GPZ_OK) return;
