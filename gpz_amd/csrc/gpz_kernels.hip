@@ -73,6 +73,8 @@ struct LaunchTimer {
 };
 
 constexpr uint32_t kGroup = 1024;
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima), gfx9 encoding
+constexpr int kWaitVmcnt0 = 0x0F70;
 constexpr uint32_t kSlot = 9344;       // max payload bytes of one group
 constexpr uint32_t kSampleLen = 256;   // src/cudaWarpCompress.cu:1425-1426
 constexpr uint32_t kSampleBlocks = 1024;
@@ -1279,26 +1281,36 @@ struct DecodeArgs {
   void* z;
 };
 
-// One wave per group: the group's payload is staged in LDS with coalesced
-// dword loads, the planes are turned back into per-run deltas / counts and
-// per-rank relative positions with the same 16x16 bit transposes the
-// encoder uses, block ids are split with per-group magic divisors, and each
-// lane writes 16 consecutive output points.
+// Point coordinate from its doubled code q2 = (q << 1) | 1
+// (src/cudaWarpCompress.cu:1003-1010): fma(q2, eb, min).
+template <typename T, typename Q>
+__device__ __forceinline__ T dequant(Q q2, T eb, T mn) {
+  return dfma(static_cast<T>(q2), eb, mn);
+}
+
+// One wave per group, reading the group's planes straight from the
+// stream.  Groups of at most 64 runs (the common case for clustered data)
+// hold one run per lane: each lane picks its bit out of every MSB-first
+// plane byte, block ids and start ranks are lane prefix sums, the block
+// coordinates of every run go to a 64-entry LDS table and every rank finds
+// its run by counting run starts in a 1024-bit map.  Larger groups undo
+// the encoder's 16x16 bit transposes (lane l: runs 16l..16l+15), keep one
+// block id per run in LDS and split it per point with per-group magic
+// divisors.  Either way lane l writes the 16 consecutive points of ranks
+// 16l..16l+15.
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+void decode_kernel(DecodeArgs a) {
   using Q = typename Prec<T>::Q;
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
-  // Staged payload; once its planes are in registers the same buffer holds
-  // the block id of every run and the run-start flags of every rank.
-  // FP32 groups with at most 512 runs keep the block coordinates of each
-  // run (already multiplied by seg) instead, so that points need no
-  // division.
-  __shared__ __align__(16) uint32_t s_pay[kSlot / 4];
-  Q* s_bid = reinterpret_cast<Q*>(s_pay);
-  uint4* s_tab = reinterpret_cast<uint4*>(s_pay);
-  uint8_t* s_start = reinterpret_cast<uint8_t*>(s_pay) + 8192;
-  static_assert(kGroup * sizeof(Q) <= 8192 && 512 * 16 <= 8192 &&
-                8192 + kGroup <= kSlot, "run tables fit the stage");
+  // [0, 1024*sizeof(Q)): block id per run, or the small-group table (four
+  // Q per run); then 1024 run-start bytes, or the small-group bitmap.
+  constexpr uint32_t kStartB = kGroup * sizeof(Q);
+  __shared__ __align__(16) uint8_t lds[kStartB + kGroup];
+  Q* s_bid = reinterpret_cast<Q*>(lds);
+  Q* s_tab = reinterpret_cast<Q*>(lds);
+  uint8_t* s_start = lds + kStartB;
+  uint32_t* s_map = reinterpret_cast<uint32_t*>(lds + kStartB);
 
   const uint32_t lane = threadIdx.x;
   const uint64_t c = blockIdx.x;
@@ -1319,119 +1331,25 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   if (U == 0 || U > kGroup || db > 32 || cb > 16 || rb > 32 ||
       pay0 + D + C + R > a.stream_bytes)
     return;
-  const uint32_t loc = D + C + R;
-
-  // ---- stage the payload (4-byte aligned) into LDS
-  {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.stream + pay0);
-    const uint32_t nw = loc >> 2;
-    for (uint32_t base = 0; base < nw; base += 512) {
-      uint32_t v[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint32_t i = base + 64u * k + lane;
-        v[k] = i < nw ? src[i] : 0u;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint32_t i = base + 64u * k + lane;
-        if (i < nw) s_pay[i] = v[k];
-      }
-    }
-  }
-  wave_sync();
-  const uint8_t* pb = reinterpret_cast<const uint8_t*>(s_pay);
+  const uint8_t* pay = a.stream + pay0;
   const uint32_t B = (U + 7u) >> 3;
 
-  const T eb = static_cast<T>(a.eb);
-  const uint32_t seg = a.seg;
-  const T seg2eb = static_cast<T>(2u * seg) * eb;
-  Q nbx, nby;
-  block_counts<T, DIM>(mm[0], mm[3], mm[1], mm[4], seg2eb, &nbx, &nby);
-  const uint64_t base = c * kGroup;
-  const uint32_t nvalid =
-      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
-
-  // ---- runs 16l..16l+15: deltas and counts from the MSB-first planes.
-  // bfrev of the two plane bytes 2l, 2l+1 gives a 16-bit word with bit t =
-  // run 16l+t; the bit transpose turns plane words back into values.
-  uint32_t del[16], cnt[16];
-  {
-    if (db <= 16) {
-      uint32_t r[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) r[i] = 0;
-#pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        if (b < static_cast<int>(db)) {
-          const uint8_t* q = pb + (db - 1u - b) * B + 2u * lane;
-          const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
-          const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
-          const uint32_t w = __brev((b0 << 24) | (b1 << 16));
-          r[b & 7] |= b < 8 ? w : (w << 16);
-        }
-      }
-      transpose16_packed(r);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        del[i] = r[i] & 0xFFFFu;
-        del[i + 8] = r[i] >> 16;
-      }
-    } else {
-      uint32_t t16[16];
-#pragma unroll
-      for (int t = 0; t < 16; ++t) t16[t] = 0;
-#pragma unroll
-      for (int b = 0; b < 32; ++b) {
-        if (b < static_cast<int>(db)) {
-          const uint8_t* q = pb + (db - 1u - b) * B + 2u * lane;
-          const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
-          const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
-          const uint32_t w = __brev((b0 << 24) | (b1 << 16));
-          t16[b & 15] |= b < 16 ? w : (w << 16);
-        }
-      }
-      transpose16x2(t16);
-#pragma unroll
-      for (int t = 0; t < 16; ++t) del[t] = t16[t];
-    }
-    uint32_t r[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) r[i] = 0;
-#pragma unroll
-    for (int b = 0; b < 16; ++b) {
-      if (b < static_cast<int>(cb)) {
-        const uint8_t* q = pb + D + (cb - 1u - b) * B + 2u * lane;
-        const uint32_t b0 = 2u * lane < B ? q[0] : 0u;
-        const uint32_t b1 = 2u * lane + 1u < B ? q[1] : 0u;
-        const uint32_t w = __brev((b0 << 24) | (b1 << 16));
-        r[b & 7] |= b < 8 ? w : (w << 16);
-      }
-    }
-    transpose16_packed(r);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      cnt[i] = r[i] & 0xFFFFu;
-      cnt[i + 8] = r[i] >> 16;
-    }
-  }
-  // ---- relative positions of ranks 16l..16l+15 (LSB-first planes)
+  // ---- relative positions of ranks 16l..16l+15 (LSB-first planes, one
+  // 16-bit word per lane and plane)
   uint32_t rel[16];
   {
-    const uint8_t* pr = pb + D + C;
+    const uint8_t* pr = pay + D + C + 2u * lane;
+    auto plane16 = [&](int b) -> uint32_t {
+      return *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u);
+    };
     if (rb <= 16) {
+      uint32_t w[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) w[b] = b < static_cast<int>(rb) ? plane16(b) : 0u;
       uint32_t r[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) r[i] = 0;
-#pragma unroll
-      for (int b = 0; b < 16; ++b) {
-        if (b < static_cast<int>(rb)) {
-          const uint32_t w =
-              *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
-          r[b & 7] |= b < 8 ? w : (w << 16);
-        }
-      }
-      transpose16_packed(r);
+      for (int i = 0; i < 8; ++i) r[i] = pack_pair(w[i], w[i + 8]);
+      transpose16_bytes(r);
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         rel[i] = r[i] & 0xFFFFu;
@@ -1443,18 +1361,22 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
 #pragma unroll
       for (int b = 0; b < 32; ++b) {
         if (b < static_cast<int>(rb)) {
-          const uint32_t w =
-              *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane);
+          const uint32_t w = plane16(b);
           rel[b & 15] |= b < 16 ? w : (w << 16);
         }
       }
       transpose16x2(rel);
     }
   }
-  // every plane is in registers: the stage buffer becomes the run tables
-  wave_sync();
-  reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
-  wave_sync();
+
+  const T eb = static_cast<T>(a.eb);
+  const uint32_t seg = a.seg;
+  const T seg2eb = static_cast<T>(2u * seg) * eb;
+  Q nbx, nby;
+  block_counts<T, DIM>(mm[0], mm[3], mm[1], mm[4], seg2eb, &nbx, &nby);
+  const uint64_t base = c * kGroup;
+  const uint32_t nvalid =
+      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
   const Q m = static_cast<Q>(seg - 1u);
   const Q sq = static_cast<Q>(seg);
@@ -1462,85 +1384,179 @@ __global__ __launch_bounds__(64) void decode_kernel(DecodeArgs a) {
   const Div32 dvx = div32_make(static_cast<uint32_t>(nbx ? nbx : 1));
   const Div32 dvy = div32_make(static_cast<uint32_t>(nby ? nby : 1));
   const Div32 dvxy = div32_make(static_cast<uint32_t>(nxy ? nxy : 1));
-  const bool table = sizeof(T) == 4 && U <= 512;
-  // block id per run = outlier + prefix of deltas (delta of run 0 ignored,
-  // src/cudaWarpCompress.cu:930-935); run end = prefix of counts.
-  Q bsum = 0;
-  uint32_t csum = 0;
-  Q bid[16];
-  uint32_t cend[16];
-#pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const uint32_t i = 16u * lane + t;
-    const Q d = (i == 0) ? outlier : (i < U ? static_cast<Q>(del[t]) : Q(0));
-    const uint32_t cn = i < U ? cnt[t] : 0u;
-    bsum += d;
-    csum += cn;
-    bid[t] = bsum;
-    cend[t] = csum;
-  }
-  const Q bincl = static_cast<Q>(wave_incl_scan(static_cast<uint64_t>(bsum), lane));
-  const uint32_t cincl = wave_incl_scan(csum, lane);
-  const Q bpre = static_cast<Q>(bincl - bsum);
-  const uint32_t cpre = cincl - csum;
-#pragma unroll
-  for (int t = 0; t < 16; ++t) {
-    const uint32_t i = 16u * lane + t;
-    if (i < U) {
-      const Q b = static_cast<Q>(bid[t] + bpre);
-      if (table) {
-        const Q qb = qdiv<Q>(b, nbx, dvx);
-        const Q bx = nbx ? b - qb * nbx : b;
-        const Q by = nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb;
-        const Q bz = qdiv<Q>(b, nxy, dvxy);
-        s_tab[i] = make_uint4(static_cast<uint32_t>(bx * sq), static_cast<uint32_t>(by * sq),
-                              static_cast<uint32_t>(bz * sq), 0u);
-      } else {
-        s_bid[i] = b;
-      }
-      const uint32_t st = cend[t] + cpre - cnt[t];
-      if (st < kGroup) s_start[st] = 1;
-    }
-  }
-  wave_sync();
-  // ---- rank -> run index by a prefix count of run starts
-  const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
-  const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
-  uint32_t rs = 0;
-  uint32_t runof[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    rs += (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-    runof[k] = rs;
-  }
-  const uint32_t rpre = wave_incl_scan(rs, lane) - rs;
+  // block coordinates of block id b, each times seg
+  auto split = [&](Q b, Q* bx, Q* by, Q* bz) {
+    const Q qb = qdiv<Q>(b, nbx, dvx);            // b / nbx
+    *bx = (nbx ? b - qb * nbx : b) * sq;          // b % nbx
+    *by = (nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb) * sq;
+    *bz = DIM == 3 ? qdiv<Q>(b, nxy, dvxy) * sq : Q(0);
+  };
+  auto inc_scan = [&](Q v) -> Q {
+    if constexpr (sizeof(Q) == 4) return wave_scan(v);
+    else return static_cast<Q>(wave_incl_scan(static_cast<uint64_t>(v), lane));
+  };
 
   T ox[16], oy[16], oz[16];
+  if (U <= 64 && db <= 16) {
+    // ---- one run per lane.  Plane bytes past the plane (u >= 8B) only
+    // feed runs >= U; they re-read byte 0 to stay inside the stream.
+    const uint32_t u = lane;
+    const bool valid = u < U;
+    const uint32_t boff = u < 8u * B ? (u >> 3) : 0u;
+    const uint32_t bsh = 7u - (u & 7u);
+    uint32_t pd[16], pc[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    uint32_t run = runof[k] + rpre;
-    run = run ? run - 1u : 0u;
-    if (run >= U) run = U - 1u;
-    const Q rl = static_cast<Q>(rel[k]);
-    Q qx, qy, qz = 0;
-    if (table) {
-      const uint4 tb = s_tab[run];
-      qx = static_cast<Q>(tb.x) + (rl & m);
-      qy = static_cast<Q>(tb.y) + (shr(rl, sh) & m);
-      if (DIM == 3) qz = static_cast<Q>(tb.z) + shr(rl, 2u * sh);
-    } else {
-      const Q b = s_bid[run];
-      const Q qb = qdiv<Q>(b, nbx, dvx);            // b / nbx
-      const Q bx = nbx ? b - qb * nbx : b;          // b % nbx
-      const Q by = nby ? qb - qdiv<Q>(qb, nby, dvy) * nby : qb;
-      qx = bx * sq + (rl & m);
-      qy = by * sq + (shr(rl, sh) & m);
-      if (DIM == 3) qz = qdiv<Q>(b, nxy, dvxy) * sq + shr(rl, 2u * sh);
+    for (int b = 0; b < 16; ++b) {
+      pd[b] = b < static_cast<int>(db) ? pay[(db - 1u - b) * B + boff] : 0u;
+      pc[b] = b < static_cast<int>(cb) ? pay[D + (cb - 1u - b) * B + boff] : 0u;
     }
-    ox[k] = dfma(static_cast<T>((qx << 1) | 1u), eb, mm[0]);
-    oy[k] = dfma(static_cast<T>((qy << 1) | 1u), eb, mm[1]);
-    if (DIM == 3) oz[k] = dfma(static_cast<T>((qz << 1) | 1u), eb, mm[2]);
+    uint32_t dv = 0, cv = 0;
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      dv |= ((pd[b] >> bsh) & 1u) << b;
+      cv |= ((pc[b] >> bsh) & 1u) << b;
+    }
+    // block id = outlier + deltas of runs 1..u (delta of run 0 ignored,
+    // src/cudaWarpCompress.cu:930-935); start rank = counts of runs < u
+    const Q dq = u == 0 ? outlier : (valid ? static_cast<Q>(dv) : Q(0));
+    const Q bid = inc_scan(dq);
+    const uint32_t cn = valid ? cv : 0u;
+    const uint32_t st = wave_scan(cn) - cn;
+    Q bx, by, bz;
+    split(bid, &bx, &by, &bz);
+    if (lane < 32) s_map[lane] = 0u;
+    if (valid) {
+      s_tab[4 * u + 0] = (bx << 1) | 1u;
+      s_tab[4 * u + 1] = (by << 1) | 1u;
+      s_tab[4 * u + 2] = (bz << 1) | 1u;
+    }
+    wave_sync();
+    if (valid && st < kGroup)
+      __hip_atomic_fetch_or(s_map + (st >> 5), 1u << (st & 31u), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
+    wave_sync();
+    // ---- rank -> run: starts at or before the rank, minus one
+    const uint32_t w = (s_map[lane >> 1] >> (16u * (lane & 1u))) & 0xFFFFu;
+    const uint32_t nw = __popc(w);
+    const uint32_t ex = wave_scan(nw) - nw - 1u;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t run = ex + __popc(w & ((2u << k) - 1u));
+      const Q rl = static_cast<Q>(rel[k]);
+      const Q* tb = s_tab + 4 * run;
+      ox[k] = dequant<T>(tb[0] + ((rl & m) << 1), eb, mm[0]);
+      oy[k] = dequant<T>(tb[1] + ((shr(rl, sh) & m) << 1), eb, mm[1]);
+      if (DIM == 3) oz[k] = dequant<T>(tb[2] + (shr(rl, 2u * sh) << 1), eb, mm[2]);
+    }
+  } else {
+    // ---- runs 16l..16l+15: lane l's two bytes of each MSB-first plane
+    // (b0 | b1 << 16) through the encoder's transpose in reverse; slot s
+    // then holds run 16l + (s^7).  Bytes past the plane (2l >= B) only feed
+    // runs >= U; they re-read byte 0 to stay inside the stream.
+    const uint32_t o0 = 2u * lane < B ? 2u * lane : 0u;
+    const uint32_t o1 = 2u * lane + 1u < B ? 2u * lane + 1u : 0u;
+    auto plane2 = [&](uint32_t off) -> uint32_t {
+      return static_cast<uint32_t>(pay[off + o0]) | (static_cast<uint32_t>(pay[off + o1]) << 16);
+    };
+    uint32_t del[16], cnt[16];
+    if (db <= 16) {
+      uint32_t w[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) w[b] = b < static_cast<int>(db) ? plane2((db - 1u - b) * B) : 0u;
+      uint32_t r[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(w[i + 8], w[i], 0x06020400u);
+      transpose16_bytes(r);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        del[7 - i] = r[i] & 0xFFFFu;
+        del[15 - i] = r[i] >> 16;
+      }
+    } else {
+      uint32_t t16[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) t16[t] = 0;
+#pragma unroll
+      for (int b = 0; b < 32; ++b) {
+        if (b < static_cast<int>(db)) {
+          const uint32_t x = plane2((db - 1u - b) * B);
+          const uint32_t w = (x & 0xFFu) | ((x >> 8) & 0xFF00u);
+          t16[b & 15] |= b < 16 ? w : (w << 16);
+        }
+      }
+      transpose16x2(t16);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) del[t ^ 7] = t16[t];
+    }
+    {
+      uint32_t w[16];
+#pragma unroll
+      for (int b = 0; b < 16; ++b) w[b] = b < static_cast<int>(cb) ? plane2(D + (cb - 1u - b) * B) : 0u;
+      uint32_t r[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(w[i + 8], w[i], 0x06020400u);
+      transpose16_bytes(r);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        cnt[7 - i] = r[i] & 0xFFFFu;
+        cnt[15 - i] = r[i] >> 16;
+      }
+    }
+    // block id per run = outlier + prefix of deltas; run end = prefix of
+    // counts
+    reinterpret_cast<uint4*>(s_start)[lane] = make_uint4(0, 0, 0, 0);
+    Q bsum = 0;
+    uint32_t csum = 0;
+    Q bid[16];
+    uint32_t cend[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const uint32_t i = 16u * lane + t;
+      const Q d = (i == 0) ? outlier : (i < U ? static_cast<Q>(del[t]) : Q(0));
+      const uint32_t cn = i < U ? cnt[t] : 0u;
+      bsum += d;
+      csum += cn;
+      bid[t] = bsum;
+      cend[t] = csum;
+    }
+    const Q bpre = inc_scan(bsum) - bsum;
+    const uint32_t cpre = wave_scan(csum) - csum;
+    wave_sync();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const uint32_t i = 16u * lane + t;
+      if (i < U) {
+        s_bid[i] = static_cast<Q>(bid[t] + bpre);
+        const uint32_t st = cend[t] + cpre - cnt[t];
+        if (st < kGroup) s_start[st] = 1;
+      }
+    }
+    wave_sync();
+    // ---- rank -> run index by a prefix count of run starts
+    const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
+    const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
+    uint32_t rs = 0;
+    uint32_t runof[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      rs += (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+      runof[k] = rs;
+    }
+    const uint32_t rpre = wave_scan(rs) - rs;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      uint32_t run = runof[k] + rpre;
+      run = run ? run - 1u : 0u;
+      if (run >= U) run = U - 1u;
+      const Q rl = static_cast<Q>(rel[k]);
+      Q bx, by, bz;
+      split(s_bid[run], &bx, &by, &bz);
+      ox[k] = dequant<T>((((bx + (rl & m)) << 1) | 1u), eb, mm[0]);
+      oy[k] = dequant<T>((((by + (shr(rl, sh) & m)) << 1) | 1u), eb, mm[1]);
+      if (DIM == 3) oz[k] = dequant<T>((((bz + shr(rl, 2u * sh)) << 1) | 1u), eb, mm[2]);
+    }
   }
+
   // ---- store ranks 16l..16l+15 of each plane
   T* X = static_cast<T*>(a.x) + base + 16u * lane;
   T* Y = static_cast<T*>(a.y) + base + 16u * lane;
