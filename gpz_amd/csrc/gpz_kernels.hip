@@ -550,6 +550,7 @@ struct EncodeArgs {
   const void* y;
   const void* z;
   uint64_t n, G;
+  uint32_t last_n;  // points in group G-1 (1..1024)
   double eb;
   const DevInfo* info;
   uint8_t* meta;    // stream meta table [G * kMeta]
@@ -811,7 +812,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   const uint32_t seg = a.info->seg_size;
   const T eb = static_cast<T>(a.eb);
   const uint64_t base = c * kGroup;
-  const uint32_t nvalid = kFull ? kGroup : static_cast<uint32_t>(a.n - base);
+  const uint32_t nvalid = kFull ? kGroup : a.last_n;
   const bool last = (c == a.G - 1);
   const uint32_t jbase = 512u * (lane & 1u) + (lane >> 1);
 
@@ -1114,7 +1115,9 @@ void encode_kernel(EncodeArgs a) {
   const uint32_t lane = threadIdx.x;
   if (a.info->status != GPZ_OK) return;
   const uint64_t c = blockIdx.x;
-  if ((c + 1) * kGroup <= a.n)
+  // The group size comes from 32-bit values: the last group's size is
+  // precomputed on the host.
+  if (blockIdx.x + 1u < static_cast<uint32_t>(a.G) || a.last_n == kGroup)
     encode_group<T, DIM, true>(a, c, lane, lds);
   else
     encode_group<T, DIM, false>(a, c, lane, lds);
@@ -1274,6 +1277,7 @@ struct DecodeArgs {
   const uint8_t* stream;
   uint64_t stream_bytes;
   uint64_t n, G;
+  uint32_t last_n;  // points in group G-1 (1..1024)
   double eb;
   uint32_t seg;
   void* x;
@@ -1376,7 +1380,7 @@ void decode_kernel(DecodeArgs a) {
   block_counts<T, DIM>(mm[0], mm[3], mm[1], mm[4], seg2eb, &nbx, &nby);
   const uint64_t base = c * kGroup;
   const uint32_t nvalid =
-      static_cast<uint32_t>(a.n - base < kGroup ? a.n - base : kGroup);
+      blockIdx.x + 1u < static_cast<uint32_t>(a.G) ? kGroup : a.last_n;
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
   const Q m = static_cast<Q>(seg - 1u);
   const Q sq = static_cast<Q>(seg);
@@ -1557,26 +1561,44 @@ void decode_kernel(DecodeArgs a) {
     }
   }
 
-  // ---- store ranks 16l..16l+15 of each plane
-  T* X = static_cast<T*>(a.x) + base + 16u * lane;
-  T* Y = static_cast<T*>(a.y) + base + 16u * lane;
-  T* Z = static_cast<T*>(a.z) + base + 16u * lane;
-  if (16u * lane + 16u <= nvalid) {
-    constexpr int V = 16 / sizeof(T);  // elements per 16-byte store
+  // ---- store ranks 16l..16l+15 of each plane.  A full group goes
+  // through LDS one axis at a time so that every store instruction writes
+  // 64 consecutive 16-byte vectors (lane l: vectors l + 64j); with lanes
+  // holding 16 consecutive points directly, each store would touch 32
+  // cache lines.  Vector i lives at i + i/16 (no bank conflicts on either
+  // side).
+  if (nvalid == kGroup) {
+    constexpr int V = 16 / sizeof(T);   // elements per 16-byte vector
+    constexpr int NV = 16 / V;          // vectors per lane and axis
+    static_assert(64 * NV * 17 / 16 * 16 <= kStartB + kGroup, "output stage fits");
+    uint4* sv = reinterpret_cast<uint4*>(lds);
+    auto put = [&](const T (&o)[16], void* dst) {
+      wave_sync();
 #pragma unroll
-    for (int k = 0; k < 16; k += V) {
-      if (sizeof(T) == 4) {
-        *reinterpret_cast<float4*>(X + k) = make_float4(ox[k], ox[k + 1], ox[k + 2], ox[k + 3]);
-        *reinterpret_cast<float4*>(Y + k) = make_float4(oy[k], oy[k + 1], oy[k + 2], oy[k + 3]);
-        if (DIM == 3)
-          *reinterpret_cast<float4*>(Z + k) = make_float4(oz[k], oz[k + 1], oz[k + 2], oz[k + 3]);
-      } else {
-        *reinterpret_cast<double2*>(X + k) = make_double2(ox[k], ox[k + 1]);
-        *reinterpret_cast<double2*>(Y + k) = make_double2(oy[k], oy[k + 1]);
-        if (DIM == 3) *reinterpret_cast<double2*>(Z + k) = make_double2(oz[k], oz[k + 1]);
+      for (int j = 0; j < NV; ++j) {
+        const uint32_t i = NV * lane + j;
+        T e[V];
+#pragma unroll
+        for (int t = 0; t < V; ++t) e[t] = o[V * j + t];
+        uint4 w;
+        memcpy(&w, e, 16);
+        sv[i + (i >> 4)] = w;
       }
-    }
+      wave_sync();
+      uint4* d4 = reinterpret_cast<uint4*>(static_cast<T*>(dst) + base);
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const uint32_t i = lane + 64u * j;
+        d4[i] = sv[i + (i >> 4)];
+      }
+    };
+    put(ox, a.x);
+    put(oy, a.y);
+    if (DIM == 3) put(oz, a.z);
   } else {
+    T* X = static_cast<T*>(a.x) + base + 16u * lane;
+    T* Y = static_cast<T*>(a.y) + base + 16u * lane;
+    T* Z = static_cast<T*>(a.z) + base + 16u * lane;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (16u * lane + k < nvalid) {
@@ -1681,6 +1703,7 @@ int compress_impl(const gpz_params* p, const void* dx, const void* dy,
   EncodeArgs ea;
   ea.x = dx; ea.y = dy; ea.z = dz;
   ea.n = n; ea.G = G; ea.eb = sa.eb;
+  ea.last_n = static_cast<uint32_t>(n - (G - 1) * kGroup);
   ea.info = info;
   ea.meta = static_cast<uint8_t*>(dstream) + HostPrec<T>::P + 8 * G;
   ea.loc = reinterpret_cast<uint64_t*>(ws + L.loc);
@@ -1729,6 +1752,7 @@ int decompress_impl(const gpz_header* h, const void* dstream,
   da.stream = static_cast<const uint8_t*>(dstream);
   da.stream_bytes = stream_bytes;
   da.n = h->n;
+  da.last_n = static_cast<uint32_t>(h->n - (h->groups - 1) * kGroup);
   da.G = h->groups;
   da.eb = h->eb;
   da.seg = h->seg_size;
@@ -1883,7 +1907,8 @@ int gpz_decompress(const gpz_header* h, const void* d_stream,
   if (!h || !d_stream || !d_x || !d_y || (h->dim == 3 && !d_z))
     return GPZ_ERR_INVALID_ARG;
   if ((h->dim != 2 && h->dim != 3) || h->n == 0 || h->seg_size == 0 ||
-      h->groups != groups_of(h->n) || stream_bytes < h->min_stream_bytes)
+      h->groups != groups_of(h->n) || h->groups > 0x7FFFFFFFull ||
+      stream_bytes < h->min_stream_bytes)
     return GPZ_ERR_CORRUPT;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (h->float_bits == 32)

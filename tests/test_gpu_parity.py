@@ -156,3 +156,37 @@ def test_large_config_full(oracle):
     for orig, dec in zip((x, y, z), d):
         assert _group_sorted_bound(_dev(orig), dec, r.eb) == 0
     assert len(s_o) == r.nbytes
+
+
+@pytest.mark.parametrize("bits,dim", [(32, 3), (32, 2), (64, 3), (64, 2)])
+@pytest.mark.parametrize("n", [1, 1023, 1025, 3 * 1024, 5000])
+def test_decode_stays_inside_outputs(oracle, bits, dim, n):
+    """Decoding into slices of a sentinel-filled buffer: every point of the
+    output is written, nothing past its end (the last group may be
+    partial)."""
+    import ctypes
+    from gpz_amd import api, datagen, _native as N
+    x, y, z = datagen.clustered(n, dim=dim, bits=bits, clusters=8, seed=n)
+    zz = z if dim == 3 else None
+    r = api.compress(_dev(x), _dev(y), _dev(zz), mode="rel", e=1e-4)
+    hdr = api.read_header(r.bytes())
+    dt = torch.float64 if bits == 64 else torch.float32
+    pad = 4096
+    big = torch.full((3 * (n + pad),), -12345.0, dtype=dt, device="cuda")
+    outs = [big[i * (n + pad): i * (n + pad) + n] for i in range(3)]
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = N.lib().gpz_decompress(ctypes.byref(hdr), ctypes.c_void_p(r.bytes().data_ptr()),
+                                ctypes.c_uint64(r.nbytes),
+                                *(ctypes.c_void_p(o.data_ptr()) for o in outs[:dim]),
+                                *([ctypes.c_void_p(0)] if dim == 2 else []), s)
+    torch.cuda.synchronize()
+    assert rc == 0
+    b = big.cpu().numpy()
+    d_o = oracle.decompress(r.to_numpy())
+    for a in range(3):
+        seg = b[a * (n + pad): (a + 1) * (n + pad)]
+        assert (seg[n:] == -12345.0).all(), f"axis {a}: write past the end"
+        if a < dim:
+            np.testing.assert_array_equal(seg[:n], d_o[a])
+        else:
+            assert (seg[:n] == -12345.0).all()
