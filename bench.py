@@ -62,19 +62,24 @@ def traffic_from_profiles(kernel, workload):
         return None
 
 
-def cpu_baseline(x, y, z, spec, eb, seg, threads, reps=2):
-    """The oracle (scalar/OpenMP C++ restatement) on the host cores."""
+def cpu_baseline(x, y, z, spec, eb, seg, threads, sample_points, reps=2):
+    """The oracle (scalar/OpenMP C++ restatement) on the host cores, over
+    the first `sample_points` points of the snapshot (whole 1024-point
+    groups, compressed with the snapshot's own eb and segment size, so they
+    are exactly the snapshot's first groups)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    import numpy as np
     O.build()
     used = O.num_threads(threads)
     dt = x.dtype.type
-    nbytes = x.nbytes * (3 if z is not None else 2)
+    m = min(x.size, max(1024, sample_points // 1024 * 1024))
+    xs, ys = x[:m], y[:m]
+    zs = z[:m] if z is not None else None
+    nbytes = xs.nbytes * (3 if zs is not None else 2)
 
     def once():
         t0 = time.perf_counter()
-        s = O.compress(x, y, z, dt(eb), seg)
+        s = O.compress(xs, ys, zs, dt(eb), seg)
         t1 = time.perf_counter()
         O.decompress(s)
         t2 = time.perf_counter()
@@ -85,9 +90,10 @@ def cpu_baseline(x, y, z, spec, eb, seg, threads, reps=2):
     tc, td = runs[len(runs) // 2]
     return dict(value=round(nbytes / (tc + td) / 1e9, 4), unit="GB/s",
                 cores=used, kind="port",
-                sample=(f"whole {spec['name']} snapshot ({x.size} points), "
-                        f"oracle compress+decompress, OpenMP {used} threads on "
-                        f"{cpu_model()}, median of {reps} after 1 warm-up"),
+                sample=(f"first {m} of the {x.size} points of {spec['name']} "
+                        f"({m // 1024} groups), oracle compress+decompress, "
+                        f"OpenMP {used} threads on {cpu_model()}, median of "
+                        f"{reps} after 1 warm-up ({tc + td:.2f} s per run)"),
                 compress_GBps=round(nbytes / tc / 1e9, 4),
                 decompress_GBps=round(nbytes / td / 1e9, 4))
 
@@ -101,6 +107,8 @@ def main():
     ap.add_argument("--n", type=int, default=None, help="override point count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 62,
+                    help="points of the snapshot the CPU baseline runs on")
     ap.add_argument("--profile-pass", action="store_true",
                     help="counter-collection run: a 1 GiB device copy (the "
                          "calibration kernel) then two compress and two "
@@ -234,7 +242,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(x, y, z, spec, eb, res.seg_size, args.cpu_threads)
+        cpu = cpu_baseline(x, y, z, spec, eb, res.seg_size, args.cpu_threads,
+                           args.cpu_sample)
         log(f"[bench] cpu baseline {cpu}")
 
     value = in_bytes / (ms_step * 1e-3) / 1e9
