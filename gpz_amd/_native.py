@@ -38,6 +38,11 @@ EXPORTS = (
     "gpz_read_header", "gpz_stream_bytes", "gpz_decompress", "gpz_min_max_workspace_bytes",
     "gpz_min_max", "gpz_rel_to_abs", "gpz_timing_enable", "gpz_timing_reset",
     "gpz_timing_read",
+    "gpz_batch_compress_workspace_bytes", "gpz_batch_decompress_workspace_bytes",
+    "gpz_batch_plan_compress", "gpz_batch_plan_decompress", "gpz_batch_run",
+    "gpz_batch_compress_results", "gpz_batch_graph_create",
+    "gpz_batch_graph_launch", "gpz_batch_graph_destroy",
+    "gpz_batch_autotune_compress", "gpz_batch_autotune_decompress",
 )
 
 
@@ -73,6 +78,31 @@ class Result(ctypes.Structure):
                 ("status", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("p", Params), ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("z", ctypes.c_void_p), ("stream", ctypes.c_void_p),
+                ("cap", ctypes.c_uint64)]
+
+
+class BatchDItem(ctypes.Structure):
+    _fields_ = [("h", Header), ("stream", ctypes.c_void_p),
+                ("bytes", ctypes.c_uint64), ("x", ctypes.c_void_p),
+                ("y", ctypes.c_void_p), ("z", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("seg_groups", ctypes.c_uint32), ("waves_per_wg", ctypes.c_uint32)]
+
+
+class BatchPlan(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint32), ("kind", ctypes.c_uint32),
+                ("seg_groups", ctypes.c_uint32), ("waves_per_wg", ctypes.c_uint32),
+                ("cls_items", ctypes.c_uint32 * 4), ("cls_first", ctypes.c_uint32 * 4),
+                ("cls_segs", ctypes.c_uint32 * 4), ("cls_seg_first", ctypes.c_uint32 * 4),
+                ("grid", ctypes.c_uint32 * 4), ("total_groups", ctypes.c_uint64),
+                ("ws_bytes", ctypes.c_uint64)]
+
+
 def _bind(L):
     vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
     P = ctypes.POINTER
@@ -92,6 +122,21 @@ def _bind(L):
         "gpz_timing_enable": ([i32], i32),
         "gpz_timing_reset": ([], i32),
         "gpz_timing_read": ([ctypes.c_char_p, P(ctypes.c_double), P(u64)], i32),
+        "gpz_batch_compress_workspace_bytes": ([P(BatchItem), u32], u64),
+        "gpz_batch_decompress_workspace_bytes": ([P(BatchDItem), u32], u64),
+        "gpz_batch_plan_compress": ([P(BatchItem), u32, P(BatchTuning), vp, u64,
+                                     P(BatchPlan), vp], i32),
+        "gpz_batch_plan_decompress": ([P(BatchDItem), u32, P(BatchTuning), vp, u64,
+                                       P(BatchPlan), vp], i32),
+        "gpz_batch_run": ([P(BatchPlan), vp, vp], i32),
+        "gpz_batch_compress_results": ([P(BatchPlan), vp, P(Result), vp], i32),
+        "gpz_batch_graph_create": ([P(BatchPlan), vp, P(vp)], i32),
+        "gpz_batch_graph_launch": ([vp, vp], i32),
+        "gpz_batch_graph_destroy": ([vp], i32),
+        "gpz_batch_autotune_compress": ([P(BatchItem), u32, vp, u64, vp, u32,
+                                         P(BatchTuning), P(ctypes.c_float)], i32),
+        "gpz_batch_autotune_decompress": ([P(BatchDItem), u32, vp, u64, vp, u32,
+                                           P(BatchTuning), P(ctypes.c_float)], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -59,7 +59,7 @@ struct LaunchTimer {
   hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
   LaunchTimer(const char* n, hipStream_t st) : name(n), s(st) {
-    if (!timing().on) return;
+    if (!timing().on || !n) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
         hipEventRecord(a, s) != hipSuccess)
       a = b = nullptr;
@@ -421,14 +421,14 @@ struct DevInfo {  // == gpz_result, lives at workspace offset 0
 // past N are skipped by the reference kernel (src/cudaWarpCompress.cu:1230)
 // and leave the zeroed scratch in place, so they contribute 0.0.
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void seg_sample_kernel(
+__device__ __forceinline__ void seg_sample_window(
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
-    uint64_t n, uint64_t stride, T* __restrict__ win) {
+    uint64_t n, uint64_t stride, T* __restrict__ win, uint32_t b) {
   __shared__ T red[4][6];
   const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const uint64_t s = static_cast<uint64_t>(blockIdx.x) * stride;
+  const uint64_t s = static_cast<uint64_t>(b) * stride;
   if (s + kSampleLen > n) {
-    if (t < 6) win[blockIdx.x * 6 + t] = T(0);
+    if (t < 6) win[b * 6 + t] = T(0);
     return;
   }
   T v[3];
@@ -449,8 +449,14 @@ __global__ __launch_bounds__(256) void seg_sample_kernel(
     T r = red[0][t];
     for (int i = 1; i < 4; ++i)
       r = (t & 1) ? dmax(r, red[i][t]) : dmin(r, red[i][t]);
-    win[blockIdx.x * 6 + t] = r;
+    win[b * 6 + t] = r;
   }
+}
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void seg_sample_kernel(
+    const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ z,
+    uint64_t n, uint64_t stride, T* __restrict__ win) {
+  seg_sample_window<T, DIM>(x, y, z, n, stride, win, blockIdx.x);
 }
 
 struct SegArgs {
@@ -465,9 +471,8 @@ struct SegArgs {
 // src/cudaWarpCompressDouble.cu:1487-1553 (FP64), evaluated in T with the
 // same operation order as the reference host code.
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void seg_decide_kernel(const T* __restrict__ win,
-                                                         SegArgs a,
-                                                         DevInfo* info) {
+__device__ __forceinline__ void seg_decide_one(const T* __restrict__ win,
+                                               const SegArgs& a, DevInfo* info) {
   __shared__ T red[4][6];
   const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
   if (!a.sampled) {
@@ -542,6 +547,12 @@ __global__ __launch_bounds__(256) void seg_decide_kernel(const T* __restrict__ w
   info->adjusted = adjusted;
   info->status = status;
   info->stream_bytes = 0;
+}
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void seg_decide_kernel(const T* __restrict__ win,
+                                                         SegArgs a,
+                                                         DevInfo* info) {
+  seg_decide_one<T, DIM>(win, a, info);
 }
 
 // ------------------------------------------------------------- encoder
@@ -1108,19 +1119,24 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 // One wave per group.  (A variant that prefetched the next group's
 // coordinates into registers measured slower: the doubled register set
 // halves occupancy.)
+// Group c of one snapshot by one wave; lds is the wave's own kLdsBytes.
+// The group size comes from 32-bit values: the last group's size is
+// precomputed on the host.
+template <typename T, int DIM>
+__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t c,
+                                           uint32_t lane, uint8_t* lds) {
+  if (c + 1u < static_cast<uint32_t>(a.G) || a.last_n == kGroup)
+    encode_group<T, DIM, true>(a, c, lane, lds);
+  else
+    encode_group<T, DIM, false>(a, c, lane, lds);
+}
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void encode_kernel(EncodeArgs a) {
   __shared__ __align__(16) uint8_t lds[kLdsBytes];
-  const uint32_t lane = threadIdx.x;
   if (a.info->status != GPZ_OK) return;
-  const uint64_t c = blockIdx.x;
-  // The group size comes from 32-bit values: the last group's size is
-  // precomputed on the host.
-  if (blockIdx.x + 1u < static_cast<uint32_t>(a.G) || a.last_n == kGroup)
-    encode_group<T, DIM, true>(a, c, lane, lds);
-  else
-    encode_group<T, DIM, false>(a, c, lane, lds);
+  encode_one<T, DIM>(a, blockIdx.x, threadIdx.x, lds);
 }
 
 // ------------------------------------------------- offsets scan + header
@@ -1171,10 +1187,32 @@ __global__ __launch_bounds__(256) void scan_reduce_kernel(ScanArgs a) {
   if (threadIdx.x == 0) a.bsum[blockIdx.x] = tot;
 }
 
+// Stream header and total size (src/cudaWarpCompress.cu:1668-1683, FP64
+// src/cudaWarpCompressDouble.cu:1667-1683); `payload` = sum of all loc.
+template <typename T>
+__device__ __forceinline__ void write_header(const ScanArgs& a, uint64_t payload) {
+  constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  a.info->stream_bytes = P + (8ull + M) * a.G + payload;
+  uint32_t* h = reinterpret_cast<uint32_t*>(a.stream);
+  h[0] = a.dim;
+  h[1] = static_cast<uint32_t>(a.n);
+  h[2] = static_cast<uint32_t>(a.n >> 32);
+  h[3] = sizeof(T) * 8;
+  if (sizeof(T) == 4) {
+    h[4] = __float_as_uint(static_cast<float>(a.eb));
+    h[5] = a.info->seg_size;
+  } else {
+    const uint64_t u = static_cast<uint64_t>(__double_as_longlong(a.eb));
+    h[4] = static_cast<uint32_t>(u);
+    h[5] = static_cast<uint32_t>(u >> 32);
+    h[6] = a.info->seg_size;
+    h[7] = 32;   // parameter_size, src/cudaWarpCompressDouble.cu:1674-1680
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void scan_top_kernel(ScanArgs a, uint32_t nb) {
   __shared__ uint64_t sh[4];
-  constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
   if (a.info->status != GPZ_OK) return;
   uint64_t carry = 0;
   for (uint32_t base = 0; base < nb; base += 256) {
@@ -1186,24 +1224,7 @@ __global__ __launch_bounds__(256) void scan_top_kernel(ScanArgs a, uint32_t nb) 
     carry += tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    a.info->stream_bytes = P + (8ull + M) * a.G + carry;
-    uint32_t* h = reinterpret_cast<uint32_t*>(a.stream);
-    h[0] = a.dim;
-    h[1] = static_cast<uint32_t>(a.n);
-    h[2] = static_cast<uint32_t>(a.n >> 32);
-    h[3] = sizeof(T) * 8;
-    if (sizeof(T) == 4) {
-      h[4] = __float_as_uint(static_cast<float>(a.eb));
-      h[5] = a.info->seg_size;
-    } else {
-      const uint64_t u = static_cast<uint64_t>(__double_as_longlong(a.eb));
-      h[4] = static_cast<uint32_t>(u);
-      h[5] = static_cast<uint32_t>(u >> 32);
-      h[6] = a.info->seg_size;
-      h[7] = 32;   // parameter_size, src/cudaWarpCompressDouble.cu:1674-1680
-    }
-  }
+  if (threadIdx.x == 0) write_header<T>(a, carry);
 }
 
 template <typename T>
@@ -1244,32 +1265,35 @@ struct CompactArgs {
 // whole groups from their fixed slots to their packed offsets, grid-stride,
 // eight dword loads in flight per lane.
 template <typename T>
-__global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
+__device__ __forceinline__ void compact_group(const CompactArgs& a, uint64_t c,
+                                              uint32_t lane) {
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
+  uint8_t* pay = a.stream + P + (8ull + M) * a.G;
+  const uint32_t words = static_cast<uint32_t>(a.loc[c]) >> 2;
+  const uint32_t* src =
+      reinterpret_cast<const uint32_t*>(a.slots + c * static_cast<uint64_t>(kSlot));
+  uint32_t* dst = reinterpret_cast<uint32_t*>(pay + a.off[c]);
+  for (uint32_t i0 = 0; i0 < words; i0 += 512) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = i0 + 64u * k + lane;
+      v[k] = i < words ? src[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = i0 + 64u * k + lane;
+      if (i < words) dst[i] = v[k];
+    }
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(256) void compact_kernel(CompactArgs a) {
   if (a.info->status != GPZ_OK) return;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x) >> 6;
   const uint64_t nwaves = static_cast<uint64_t>(gridDim.x) * 4;
-  uint8_t* pay = a.stream + P + (8ull + M) * a.G;
-  for (uint64_t c = wave; c < a.G; c += nwaves) {
-    const uint32_t words = static_cast<uint32_t>(a.loc[c]) >> 2;
-    const uint32_t* src =
-        reinterpret_cast<const uint32_t*>(a.slots + c * static_cast<uint64_t>(kSlot));
-    uint32_t* dst = reinterpret_cast<uint32_t*>(pay + a.off[c]);
-    for (uint32_t i0 = 0; i0 < words; i0 += 512) {
-      uint32_t v[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint32_t i = i0 + 64u * k + lane;
-        v[k] = i < words ? src[i] : 0u;
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const uint32_t i = i0 + 64u * k + lane;
-        if (i < words) dst[i] = v[k];
-      }
-    }
-  }
+  for (uint64_t c = wave; c < a.G; c += nwaves) compact_group<T>(a, c, lane);
 }
 
 // ------------------------------------------------------------- decoder
@@ -1302,22 +1326,25 @@ __device__ __forceinline__ T dequant(Q q2, T eb, T mn) {
 // block id per run in LDS and split it per point with per-group magic
 // divisors.  Either way lane l writes the 16 consecutive points of ranks
 // 16l..16l+15.
+// LDS of one decode wave: [0, 1024*sizeof(Q)) block id per run, or the
+// small-group table (four Q per run); then 1024 run-start bytes, or the
+// small-group bitmap.  The output stage reuses all of it.
+template <typename T> constexpr uint32_t dec_lds_bytes() {
+  return kGroup * sizeof(typename Prec<T>::Q) + kGroup;
+}
+
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
-void decode_kernel(DecodeArgs a) {
+__device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
+                                             uint32_t lane, uint8_t* lds) {
   using Q = typename Prec<T>::Q;
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
-  // [0, 1024*sizeof(Q)): block id per run, or the small-group table (four
-  // Q per run); then 1024 run-start bytes, or the small-group bitmap.
   constexpr uint32_t kStartB = kGroup * sizeof(Q);
-  __shared__ __align__(16) uint8_t lds[kStartB + kGroup];
   Q* s_bid = reinterpret_cast<Q*>(lds);
   Q* s_tab = reinterpret_cast<Q*>(lds);
   uint8_t* s_start = lds + kStartB;
   uint32_t* s_map = reinterpret_cast<uint32_t*>(lds + kStartB);
 
-  const uint32_t lane = threadIdx.x;
-  const uint64_t c = blockIdx.x;
+  const uint64_t c = c32;
   const uint8_t* meta = a.stream + P + 8 * a.G + M * c;
   T mm[6];
 #pragma unroll
@@ -1380,7 +1407,7 @@ void decode_kernel(DecodeArgs a) {
   block_counts<T, DIM>(mm[0], mm[3], mm[1], mm[4], seg2eb, &nbx, &nby);
   const uint64_t base = c * kGroup;
   const uint32_t nvalid =
-      blockIdx.x + 1u < static_cast<uint32_t>(a.G) ? kGroup : a.last_n;
+      c32 + 1u < static_cast<uint32_t>(a.G) ? kGroup : a.last_n;
   const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
   const Q m = static_cast<Q>(seg - 1u);
   const Q sq = static_cast<Q>(seg);
@@ -1610,6 +1637,162 @@ void decode_kernel(DecodeArgs a) {
   }
 }
 
+template <typename T, int DIM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+void decode_kernel(DecodeArgs a) {
+  __shared__ __align__(16) uint8_t lds[dec_lds_bytes<T>()];
+  decode_group<T, DIM>(a, blockIdx.x, threadIdx.x, lds);
+}
+
+// ------------------------------------------------------------- batches
+// Many snapshots of uneven size, dimension and precision per call.  For
+// each (precision, dimension) class the groups of all its snapshots form a
+// device-side queue of segments (snapshot, first group, group count);
+// persistent waves take segments w, w + waves, w + 2*waves, ... so that a
+// launch covers every snapshot of the class.  The per-group work is the
+// per-call path's own (encode_one, compact_group, decode_group), so a
+// snapshot's stream is the same whether it is compressed alone or in a
+// batch.
+
+struct BatchItem {    // device descriptor of one snapshot to compress
+  EncodeArgs e;
+  ScanArgs sc;
+  CompactArgs ca;
+  SegArgs sa;
+  DevInfo* info;
+  void* win;
+  uint64_t stride;    // sample window stride
+};
+
+struct Segment {
+  uint32_t item, g0, ng, pad;
+};
+
+// Classes: 0 = f32 2D, 1 = f32 3D, 2 = f64 2D, 3 = f64 3D.
+constexpr int kClasses = 4;
+__host__ __device__ inline int class_of(uint32_t bits, uint32_t dim) {
+  return (bits == 64 ? 2 : 0) + (dim == 3 ? 1 : 0);
+}
+
+__device__ __forceinline__ uint32_t wave_in_block() {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void batch_seg_sample_kernel(const BatchItem* items,
+                                                               const uint32_t* cls) {
+  const BatchItem& it = items[cls[blockIdx.y]];
+  if (!it.sa.sampled) return;
+  seg_sample_window<T, DIM>(static_cast<const T*>(it.e.x), static_cast<const T*>(it.e.y),
+                            static_cast<const T*>(it.e.z), it.e.n, it.stride,
+                            static_cast<T*>(it.win), blockIdx.x);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void batch_seg_decide_kernel(const BatchItem* items,
+                                                               const uint32_t* cls) {
+  const BatchItem& it = items[cls[blockIdx.x]];
+  seg_decide_one<T, DIM>(static_cast<const T*>(it.win), it.sa, it.info);
+}
+
+// The lane index, the LDS slice and the descriptor pointer are made opaque
+// in every iteration of a persistent loop: otherwise the compiler hoists
+// every lane-dependent value of the group body out of the loop and spills.
+#define GPZ_OPAQUE_ITER(ap, ln, lp) asm volatile("" : "+s"(ap), "+v"(ln), "+s"(lp))
+
+// Up to 8 waves per workgroup, each with its own kLdsBytes of dynamic LDS.
+// Three waves per SIMD: the loop state does not fit the 128 registers of
+// the single-group kernel without spilling.
+template <typename T, int DIM>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+void batch_encode_kernel(const BatchItem* __restrict__ items,
+                         const Segment* __restrict__ segs, uint32_t nsegs) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const uint32_t lane = threadIdx.x & 63u, w = wave_in_block();
+  const uint32_t nw = blockDim.x >> 6;
+  uint8_t* lds = dyn_lds + w * kLdsBytes;
+  for (uint32_t s = blockIdx.x * nw + w; s < nsegs; s += gridDim.x * nw) {
+    const Segment sg = segs[s];
+    for (uint32_t g = sg.g0; g < sg.g0 + sg.ng; ++g) {
+      const EncodeArgs* ap = &items[sg.item].e;
+      uint32_t ln = lane;
+      uint8_t* lp = lds;
+      GPZ_OPAQUE_ITER(ap, ln, lp);
+      if (ap->info->status != GPZ_OK) break;
+      encode_one<T, DIM>(*ap, g, ln, lp);
+    }
+  }
+}
+
+// One workgroup per snapshot: offsets by a blocked exclusive scan of its
+// payload sizes, then the header.
+template <typename T>
+__global__ __launch_bounds__(256) void batch_scan_kernel(const BatchItem* items,
+                                                         const uint32_t* cls) {
+  __shared__ uint64_t sh[4];
+  constexpr uint32_t P = Prec<T>::kHeader;
+  const ScanArgs& a = items[cls[blockIdx.x]].sc;
+  if (a.info->status != GPZ_OK) return;
+  uint64_t* soff = reinterpret_cast<uint64_t*>(a.stream + P);
+  uint64_t carry = 0;
+  for (uint64_t base = 0; base < a.G; base += kScanBlock) {
+    const uint64_t i0 = base + 4 * threadIdx.x;
+    uint64_t v[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v[i] = i0 + i < a.G ? a.loc[i0 + i] : 0;
+      sum += v[i];
+    }
+    uint64_t tot;
+    uint64_t run = carry + block_excl_scan256(sum, &tot, sh);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i0 + i < a.G) {
+        a.off[i0 + i] = run;
+        soff[i0 + i] = run;
+      }
+      run += v[i];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) write_header<T>(a, carry);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void batch_compact_kernel(const BatchItem* items,
+                                                            const Segment* segs,
+                                                            uint32_t nsegs) {
+  const uint32_t lane = threadIdx.x & 63u, w = wave_in_block();
+  for (uint32_t s = blockIdx.x * 4 + w; s < nsegs; s += gridDim.x * 4) {
+    const Segment sg = segs[s];
+    const CompactArgs& a = items[sg.item].ca;
+    if (a.info->status != GPZ_OK) continue;
+    for (uint32_t g = sg.g0; g < sg.g0 + sg.ng; ++g) compact_group<T>(a, g, lane);
+  }
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void batch_decode_kernel(const DecodeArgs* __restrict__ items,
+                         const Segment* __restrict__ segs, uint32_t nsegs) {
+  extern __shared__ __align__(16) uint8_t dyn_lds[];
+  const uint32_t lane = threadIdx.x & 63u, w = wave_in_block();
+  const uint32_t nw = blockDim.x >> 6;
+  uint8_t* lds = dyn_lds + w * dec_lds_bytes<T>();
+  for (uint32_t s = blockIdx.x * nw + w; s < nsegs; s += gridDim.x * nw) {
+    const Segment sg = segs[s];
+    for (uint32_t g = sg.g0; g < sg.g0 + sg.ng; ++g) {
+      const DecodeArgs* ap = &items[sg.item];
+      uint32_t ln = lane;
+      uint8_t* lp = lds;
+      GPZ_OPAQUE_ITER(ap, ln, lp);
+      decode_group<T, DIM>(*ap, g, ln, lp);
+    }
+  }
+}
+
 // ---------------------------------------------------------- min / max
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void minmax_kernel(const T* __restrict__ x,
@@ -1789,6 +1972,181 @@ int minmax_impl(uint64_t n, const void* dx, const void* dy, const void* dz,
                     r[1], r[3], DIM == 3 ? r[5] : T(0)};
   memcpy(mm, out, sizeof(out));
   return GPZ_OK;
+}
+
+// ---- host side of batches
+int device_cus() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) n = 0;
+  cache[dev] = n;
+  return n;
+}
+
+// Workgroups of `threads` that fill the device for kernel f.
+uint32_t fill_blocks(const void* f, uint32_t threads, size_t lds) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, static_cast<int>(threads), lds) !=
+          hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int cus = device_cus();
+  return static_cast<uint32_t>(per_cu * (cus > 0 ? cus : 1));
+}
+
+struct BatchLayout {
+  uint64_t items, cls, segs, infos, regions, total;
+};
+uint64_t align256h(uint64_t v) { return (v + 255) & ~255ull; }
+
+template <typename Item>
+BatchLayout batch_layout(uint32_t count, uint64_t total_groups, uint64_t region_bytes) {
+  BatchLayout L;
+  L.items = 0;
+  L.cls = align256h(sizeof(Item) * count);
+  L.segs = align256h(L.cls + 4ull * count);
+  L.infos = align256h(L.segs + sizeof(Segment) * total_groups);
+  L.regions = align256h(L.infos + sizeof(DevInfo) * count);
+  L.total = L.regions + region_bytes;
+  return L;
+}
+
+bool tuning_ok(const gpz_batch_tuning* t) {
+  if (!t) return true;
+  const uint32_t s = t->seg_groups, w = t->waves_per_wg;
+  return (s == 1 || s == 2 || s == 4 || s == 8) && (w == 1 || w == 2 || w == 4 || w == 8);
+}
+
+gpz_batch_tuning default_tuning() {
+  gpz_batch_tuning t;
+  t.seg_groups = 1;
+  t.waves_per_wg = 1;
+  return t;
+}
+
+// Class lists and segment queues shared by both directions.
+template <typename GetClass, typename GetGroups>
+void build_queues(uint32_t count, uint32_t seg_groups, GetClass cls_of, GetGroups groups_of_item,
+                  gpz_batch_plan* plan, std::vector<uint32_t>* cls,
+                  std::vector<Segment>* segs) {
+  cls->clear();
+  segs->clear();
+  for (int k = 0; k < kClasses; ++k) {
+    plan->cls_first[k] = static_cast<uint32_t>(cls->size());
+    plan->cls_seg_first[k] = static_cast<uint32_t>(segs->size());
+    for (uint32_t i = 0; i < count; ++i) {
+      if (cls_of(i) != k) continue;
+      cls->push_back(i);
+      const uint64_t G = groups_of_item(i);
+      for (uint64_t g = 0; g < G; g += seg_groups) {
+        Segment sg;
+        sg.item = i;
+        sg.g0 = static_cast<uint32_t>(g);
+        sg.ng = static_cast<uint32_t>(G - g < seg_groups ? G - g : seg_groups);
+        sg.pad = 0;
+        segs->push_back(sg);
+      }
+    }
+    plan->cls_items[k] = static_cast<uint32_t>(cls->size()) - plan->cls_first[k];
+    plan->cls_segs[k] = static_cast<uint32_t>(segs->size()) - plan->cls_seg_first[k];
+  }
+}
+
+int upload(void* d_ws, const std::vector<uint8_t>& host, hipStream_t s) {
+  if (hipMemcpyAsync(d_ws, host.data(), host.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPZ_ERR_HIP;
+  return GPZ_OK;
+}
+
+template <typename T, int DIM>
+void batch_compress_class(const gpz_batch_plan* plan, int k, uint8_t* ws, hipStream_t s,
+                          bool timed) {
+  const uint32_t ni = plan->cls_items[k], ns = plan->cls_segs[k];
+  if (!ni) return;
+  const BatchLayout L = batch_layout<BatchItem>(plan->count, plan->total_groups, 0);
+  const BatchItem* items = reinterpret_cast<const BatchItem*>(ws + L.items);
+  const uint32_t* cls = reinterpret_cast<const uint32_t*>(ws + L.cls) + plan->cls_first[k];
+  const Segment* segs = reinterpret_cast<const Segment*>(ws + L.segs) + plan->cls_seg_first[k];
+  const uint32_t W = plan->waves_per_wg;
+  {
+    LaunchTimer lt(timed ? "batch_seg" : nullptr, s);
+    hipLaunchKernelGGL((batch_seg_sample_kernel<T, DIM>), dim3(kSampleBlocks, ni), dim3(256), 0,
+                       s, items, cls);
+    hipLaunchKernelGGL((batch_seg_decide_kernel<T, DIM>), dim3(ni), dim3(256), 0, s, items, cls);
+  }
+  {
+    LaunchTimer lt(timed ? "batch_encode" : nullptr, s);
+    const size_t lds = static_cast<size_t>(W) * kLdsBytes;
+    hipLaunchKernelGGL((batch_encode_kernel<T, DIM>), dim3(plan->grid[k]), dim3(64 * W), lds, s,
+                       items, segs, ns);
+  }
+  {
+    LaunchTimer lt(timed ? "batch_scan" : nullptr, s);
+    hipLaunchKernelGGL((batch_scan_kernel<T>), dim3(ni), dim3(256), 0, s, items, cls);
+  }
+  {
+    LaunchTimer lt(timed ? "batch_compact" : nullptr, s);
+    const uint32_t need = (ns + 3) / 4, fill = 4096;
+    hipLaunchKernelGGL((batch_compact_kernel<T>), dim3(need < fill ? need : fill), dim3(256), 0,
+                       s, items, segs, ns);
+  }
+}
+
+template <typename T, int DIM>
+void batch_decompress_class(const gpz_batch_plan* plan, int k, uint8_t* ws, hipStream_t s,
+                            bool timed) {
+  const uint32_t ns = plan->cls_segs[k];
+  if (!plan->cls_items[k]) return;
+  const BatchLayout L = batch_layout<DecodeArgs>(plan->count, plan->total_groups, 0);
+  const DecodeArgs* items = reinterpret_cast<const DecodeArgs*>(ws + L.items);
+  const Segment* segs = reinterpret_cast<const Segment*>(ws + L.segs) + plan->cls_seg_first[k];
+  const uint32_t W = plan->waves_per_wg;
+  LaunchTimer lt(timed ? "batch_decode" : nullptr, s);
+  const size_t lds = static_cast<size_t>(W) * dec_lds_bytes<T>();
+  hipLaunchKernelGGL((batch_decode_kernel<T, DIM>), dim3(plan->grid[k]), dim3(64 * W), lds, s,
+                     items, segs, ns);
+}
+
+// Persistent grid of class k: enough workgroups for every segment, at
+// most what fills the device.
+template <typename T, int DIM>
+uint32_t plan_grid(uint32_t kind, uint32_t W, uint32_t ns) {
+  const uint32_t fill = kind == 0
+      ? fill_blocks(reinterpret_cast<const void*>(batch_encode_kernel<T, DIM>), 64 * W,
+                    static_cast<size_t>(W) * kLdsBytes)
+      : fill_blocks(reinterpret_cast<const void*>(batch_decode_kernel<T, DIM>), 64 * W,
+                    static_cast<size_t>(W) * dec_lds_bytes<T>());
+  const uint32_t need = (ns + W - 1) / W;
+  return need < fill ? (need ? need : 1u) : fill;
+}
+void plan_grids(gpz_batch_plan* plan) {
+  const uint32_t W = plan->waves_per_wg, kind = plan->kind;
+  plan->grid[0] = plan_grid<float, 2>(kind, W, plan->cls_segs[0]);
+  plan->grid[1] = plan_grid<float, 3>(kind, W, plan->cls_segs[1]);
+  plan->grid[2] = plan_grid<double, 2>(kind, W, plan->cls_segs[2]);
+  plan->grid[3] = plan_grid<double, 3>(kind, W, plan->cls_segs[3]);
+}
+
+int batch_run_impl(const gpz_batch_plan* plan, void* d_ws, hipStream_t s, bool timed) {
+  uint8_t* ws = static_cast<uint8_t*>(d_ws);
+  if (plan->kind == 0) {
+    batch_compress_class<float, 2>(plan, 0, ws, s, timed);
+    batch_compress_class<float, 3>(plan, 1, ws, s, timed);
+    batch_compress_class<double, 2>(plan, 2, ws, s, timed);
+    batch_compress_class<double, 3>(plan, 3, ws, s, timed);
+  } else {
+    batch_decompress_class<float, 2>(plan, 0, ws, s, timed);
+    batch_decompress_class<float, 3>(plan, 1, ws, s, timed);
+    batch_decompress_class<double, 2>(plan, 2, ws, s, timed);
+    batch_decompress_class<double, 3>(plan, 3, ws, s, timed);
+  }
+  return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
 }
 
 }  // namespace
@@ -1989,6 +2347,287 @@ double gpz_rel_to_abs(uint32_t float_bits, double e, const void* mm) {
   double r = fmax(v[3] - v[0], v[4] - v[1]);
   r = fmax(r, v[5] - v[2]);
   return r * e;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- batches
+extern "C" {
+
+uint64_t gpz_batch_compress_workspace_bytes(const gpz_batch_item* items, uint32_t count) {
+  if (!items || count == 0) return 0;
+  uint64_t groups = 0, regions = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const gpz_params& p = items[i].p;
+    if (p.n == 0) return 0;
+    const uint64_t G = groups_of(p.n);
+    groups += G;
+    regions += align256h(ws_layout(p.float_bits == 64 ? 64 : 32, G).total);
+  }
+  return batch_layout<BatchItem>(count, groups, regions).total;
+}
+
+int gpz_batch_plan_compress(const gpz_batch_item* items, uint32_t count,
+                            const gpz_batch_tuning* tune, void* d_ws, uint64_t ws_bytes,
+                            gpz_batch_plan* plan, gpz_stream_t stream) {
+  if (!items || count == 0 || count > 65535 || !plan || !tuning_ok(tune))
+    return GPZ_ERR_INVALID_ARG;
+  uint64_t groups = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const gpz_batch_item& it = items[i];
+    if (!params_ok(&it.p) || !it.x || !it.y || (it.p.dim == 3 && !it.z) || !it.stream)
+      return GPZ_ERR_INVALID_ARG;
+    if (it.cap < gpz_stream_bound(&it.p)) return GPZ_ERR_CAPACITY;
+    groups += groups_of(it.p.n);
+  }
+  if (groups > 0xFFFFFFFFull) return GPZ_ERR_INVALID_ARG;
+  const uint64_t need = gpz_batch_compress_workspace_bytes(items, count);
+  if (!d_ws || ws_bytes < need) return GPZ_ERR_WORKSPACE;
+  const gpz_batch_tuning t = tune ? *tune : default_tuning();
+  memset(plan, 0, sizeof(*plan));
+  plan->count = count;
+  plan->kind = 0;
+  plan->seg_groups = t.seg_groups;
+  plan->waves_per_wg = t.waves_per_wg;
+  plan->total_groups = groups;
+  plan->ws_bytes = need;
+
+  std::vector<uint32_t> cls;
+  std::vector<Segment> segs;
+  build_queues(
+      count, t.seg_groups,
+      [&](uint32_t i) { return class_of(items[i].p.float_bits, items[i].p.dim); },
+      [&](uint32_t i) { return groups_of(items[i].p.n); }, plan, &cls, &segs);
+  const BatchLayout L = batch_layout<BatchItem>(count, groups, 0);
+  std::vector<uint8_t> host(L.regions, 0);
+  uint8_t* ws = static_cast<uint8_t*>(d_ws);
+  DevInfo* infos = reinterpret_cast<DevInfo*>(ws + L.infos);
+  uint64_t r = L.regions;
+  for (uint32_t i = 0; i < count; ++i) {
+    const gpz_batch_item& it = items[i];
+    const gpz_params& p = it.p;
+    const bool f64 = p.float_bits == 64;
+    const uint64_t n = p.n, G = groups_of(n);
+    const WsLayout W = ws_layout(f64 ? 64 : 32, G);
+    uint8_t* region = ws + r;
+    r += align256h(W.total);
+    BatchItem b;
+    memset(&b, 0, sizeof(b));
+    const double eb = f64 ? p.eb : static_cast<double>(static_cast<float>(p.eb));
+    const double rel = f64 ? p.rel_eb : static_cast<double>(static_cast<float>(p.rel_eb));
+    // FP32 always samples (its block-count guard can double any seg); FP64
+    // only to choose one (as compress_impl).
+    const bool sample = !f64 || (p.seg_size == 0 && p.mode == GPZ_MODE_REL);
+    uint8_t* stream8 = static_cast<uint8_t*>(it.stream);
+    const uint32_t P = f64 ? 32 : 24;
+    b.info = infos + i;
+    b.win = region + W.win;
+    {
+      uint64_t st = (n - kSampleLen) / kSampleBlocks;  // u64 wrap for n < 256
+      b.stride = st < 1 ? 1 : st;
+    }
+    b.sa.seg_in = p.seg_size;
+    b.sa.is_rel = p.mode == GPZ_MODE_REL;
+    b.sa.rel_small = rel <= 1e-4;
+    b.sa.sampled = sample;
+    b.sa.eb = eb;
+    b.e.x = it.x; b.e.y = it.y; b.e.z = it.z;
+    b.e.n = n; b.e.G = G; b.e.eb = eb;
+    b.e.last_n = static_cast<uint32_t>(n - (G - 1) * kGroup);
+    b.e.info = b.info;
+    b.e.meta = stream8 + P + 8 * G;
+    b.e.loc = reinterpret_cast<uint64_t*>(region + W.loc);
+    b.e.slots = region + W.slots;
+    b.sc.loc = b.e.loc;
+    b.sc.off = reinterpret_cast<uint64_t*>(region + W.off);
+    b.sc.bsum = reinterpret_cast<uint64_t*>(region + W.bsum);
+    b.sc.stream = stream8;
+    b.sc.info = b.info;
+    b.sc.n = n; b.sc.G = G; b.sc.dim = p.dim; b.sc.eb = eb;
+    b.ca.loc = b.e.loc;
+    b.ca.off = b.sc.off;
+    b.ca.slots = b.e.slots;
+    b.ca.stream = stream8;
+    b.ca.info = b.info;
+    b.ca.G = G;
+    memcpy(host.data() + L.items + sizeof(BatchItem) * i, &b, sizeof(b));
+  }
+  memcpy(host.data() + L.cls, cls.data(), 4 * cls.size());
+  memcpy(host.data() + L.segs, segs.data(), sizeof(Segment) * segs.size());
+  plan_grids(plan);
+  return upload(d_ws, host, reinterpret_cast<hipStream_t>(stream));
+}
+
+uint64_t gpz_batch_decompress_workspace_bytes(const gpz_batch_ditem* items, uint32_t count) {
+  if (!items || count == 0) return 0;
+  uint64_t groups = 0;
+  for (uint32_t i = 0; i < count; ++i) groups += items[i].h.groups;
+  return batch_layout<DecodeArgs>(count, groups, 0).total;
+}
+
+int gpz_batch_plan_decompress(const gpz_batch_ditem* items, uint32_t count,
+                              const gpz_batch_tuning* tune, void* d_ws, uint64_t ws_bytes,
+                              gpz_batch_plan* plan, gpz_stream_t stream) {
+  if (!items || count == 0 || count > 65535 || !plan || !tuning_ok(tune))
+    return GPZ_ERR_INVALID_ARG;
+  uint64_t groups = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const gpz_batch_ditem& it = items[i];
+    const gpz_header* h = &it.h;
+    if (!it.stream || !it.x || !it.y || (h->dim == 3 && !it.z)) return GPZ_ERR_INVALID_ARG;
+    if ((h->dim != 2 && h->dim != 3) || (h->float_bits != 32 && h->float_bits != 64) ||
+        h->n == 0 || h->seg_size == 0 || h->groups != groups_of(h->n) ||
+        h->groups > 0x7FFFFFFFull || it.bytes < h->min_stream_bytes)
+      return GPZ_ERR_CORRUPT;
+    groups += h->groups;
+  }
+  if (groups > 0xFFFFFFFFull) return GPZ_ERR_INVALID_ARG;
+  const uint64_t need = gpz_batch_decompress_workspace_bytes(items, count);
+  if (!d_ws || ws_bytes < need) return GPZ_ERR_WORKSPACE;
+  const gpz_batch_tuning t = tune ? *tune : default_tuning();
+  memset(plan, 0, sizeof(*plan));
+  plan->count = count;
+  plan->kind = 1;
+  plan->seg_groups = t.seg_groups;
+  plan->waves_per_wg = t.waves_per_wg;
+  plan->total_groups = groups;
+  plan->ws_bytes = need;
+  std::vector<uint32_t> cls;
+  std::vector<Segment> segs;
+  build_queues(
+      count, t.seg_groups,
+      [&](uint32_t i) { return class_of(items[i].h.float_bits, items[i].h.dim); },
+      [&](uint32_t i) { return items[i].h.groups; }, plan, &cls, &segs);
+  const BatchLayout L = batch_layout<DecodeArgs>(count, groups, 0);
+  std::vector<uint8_t> host(L.regions, 0);
+  for (uint32_t i = 0; i < count; ++i) {
+    const gpz_batch_ditem& it = items[i];
+    DecodeArgs d;
+    memset(&d, 0, sizeof(d));
+    d.stream = static_cast<const uint8_t*>(it.stream);
+    d.stream_bytes = it.bytes;
+    d.n = it.h.n;
+    d.G = it.h.groups;
+    d.last_n = static_cast<uint32_t>(it.h.n - (it.h.groups - 1) * kGroup);
+    d.eb = it.h.eb;
+    d.seg = it.h.seg_size;
+    d.x = it.x; d.y = it.y; d.z = it.z;
+    memcpy(host.data() + L.items + sizeof(DecodeArgs) * i, &d, sizeof(d));
+  }
+  memcpy(host.data() + L.cls, cls.data(), 4 * cls.size());
+  memcpy(host.data() + L.segs, segs.data(), sizeof(Segment) * segs.size());
+  plan_grids(plan);
+  return upload(d_ws, host, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gpz_batch_run(const gpz_batch_plan* plan, void* d_ws, gpz_stream_t stream) {
+  if (!plan || !d_ws || plan->count == 0) return GPZ_ERR_INVALID_ARG;
+  return batch_run_impl(plan, d_ws, reinterpret_cast<hipStream_t>(stream), true);
+}
+
+int gpz_batch_compress_results(const gpz_batch_plan* plan, const void* d_ws, gpz_result* out,
+                               gpz_stream_t stream) {
+  if (!plan || plan->kind != 0 || !d_ws || !out) return GPZ_ERR_INVALID_ARG;
+  const BatchLayout L = batch_layout<BatchItem>(plan->count, plan->total_groups, 0);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (hipMemcpyAsync(out, static_cast<const uint8_t*>(d_ws) + L.infos,
+                     sizeof(gpz_result) * plan->count, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return GPZ_ERR_HIP;
+  return GPZ_OK;
+}
+
+int gpz_batch_graph_create(const gpz_batch_plan* plan, void* d_ws, void** graph_exec) {
+  if (!plan || !d_ws || !graph_exec || plan->count == 0) return GPZ_ERR_INVALID_ARG;
+  *graph_exec = nullptr;
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return GPZ_ERR_HIP;
+  int rc = GPZ_OK;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = GPZ_ERR_HIP;
+  } else {
+    const int r1 = batch_run_impl(plan, d_ws, cs, false);
+    if (hipStreamEndCapture(cs, &g) != hipSuccess || r1 != GPZ_OK) rc = GPZ_ERR_HIP;
+  }
+  hipGraphExec_t e = nullptr;
+  if (rc == GPZ_OK && hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess)
+    rc = GPZ_ERR_HIP;
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cs);
+  if (rc == GPZ_OK) *graph_exec = e;
+  return rc;
+}
+
+int gpz_batch_graph_launch(void* graph_exec, gpz_stream_t stream) {
+  if (!graph_exec) return GPZ_ERR_INVALID_ARG;
+  return hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec),
+                        reinterpret_cast<hipStream_t>(stream)) == hipSuccess
+             ? GPZ_OK
+             : GPZ_ERR_HIP;
+}
+
+int gpz_batch_graph_destroy(void* graph_exec) {
+  if (!graph_exec) return GPZ_ERR_INVALID_ARG;
+  return hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)) == hipSuccess
+             ? GPZ_OK
+             : GPZ_ERR_HIP;
+}
+
+// Sweeps groups per segment {1, 2, 4, 8} x waves per workgroup {1, 2, 4,
+// 8}: plans, runs once to warm up, then times `reps` runs with hipEvents.
+// table[4*i + j] receives the ms per run of seg_groups 2^i, waves 2^j.
+static int autotune_common(bool compress, const void* items, uint32_t count, void* d_ws,
+                           uint64_t ws_bytes, gpz_stream_t stream, uint32_t reps,
+                           gpz_batch_tuning* best, float* table) {
+  if (!best || reps == 0) return GPZ_ERR_INVALID_ARG;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return GPZ_ERR_HIP;
+  float best_ms = -1.0f;
+  int rc = GPZ_OK;
+  for (uint32_t i = 0; i < 4 && rc == GPZ_OK; ++i) {
+    for (uint32_t j = 0; j < 4 && rc == GPZ_OK; ++j) {
+      gpz_batch_tuning t;
+      t.seg_groups = 1u << i;
+      t.waves_per_wg = 1u << j;
+      gpz_batch_plan plan;
+      rc = compress ? gpz_batch_plan_compress(static_cast<const gpz_batch_item*>(items), count,
+                                              &t, d_ws, ws_bytes, &plan, stream)
+                    : gpz_batch_plan_decompress(static_cast<const gpz_batch_ditem*>(items),
+                                                count, &t, d_ws, ws_bytes, &plan, stream);
+      if (rc != GPZ_OK) break;
+      rc = batch_run_impl(&plan, d_ws, s, false);
+      if (rc != GPZ_OK) break;
+      (void)hipEventRecord(e0, s);
+      for (uint32_t k = 0; k < reps && rc == GPZ_OK; ++k) rc = batch_run_impl(&plan, d_ws, s, false);
+      (void)hipEventRecord(e1, s);
+      float ms = 0.0f;
+      if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        rc = GPZ_ERR_HIP;
+      ms /= static_cast<float>(reps);
+      if (table) table[4 * i + j] = ms;
+      if (rc == GPZ_OK && (best_ms < 0.0f || ms < best_ms)) {
+        best_ms = ms;
+        *best = t;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int gpz_batch_autotune_compress(const gpz_batch_item* items, uint32_t count, void* d_ws,
+                                uint64_t ws_bytes, gpz_stream_t stream, uint32_t reps,
+                                gpz_batch_tuning* best, float* table) {
+  return autotune_common(true, items, count, d_ws, ws_bytes, stream, reps, best, table);
+}
+
+int gpz_batch_autotune_decompress(const gpz_batch_ditem* items, uint32_t count, void* d_ws,
+                                  uint64_t ws_bytes, gpz_stream_t stream, uint32_t reps,
+                                  gpz_batch_tuning* best, float* table) {
+  return autotune_common(false, items, count, d_ws, ws_bytes, stream, reps, best, table);
 }
 
 }  // extern "C"

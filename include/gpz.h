@@ -148,6 +148,84 @@ int gpz_timing_enable(int on);
 int gpz_timing_reset(void);
 int gpz_timing_read(const char* kernel, double* total_ms, uint64_t* launches);
 
+/* ---------------------------------------------------------------- batches
+ * Many snapshots of uneven size, dimension and precision in one launch per
+ * (precision, dimension) class.  No counterpart in the reference, whose
+ * compressor handles one snapshot per call (src/cudaWarpCompress.cu:1573-1741,
+ * 1743-1856).  Each snapshot's stream is byte-identical to what
+ * gpz_compress writes for it alone.
+ *
+ * Use: gpz_batch_plan_* uploads the snapshot table and the segment queues
+ * into the workspace (synchronous); gpz_batch_run then only launches
+ * kernels, so it can be captured in a hipGraph (gpz_batch_graph_create)
+ * and replayed while the planned buffers stay valid.  Calls that overlap
+ * on one workspace race. */
+typedef struct gpz_batch_item {   /* one snapshot to compress */
+  gpz_params p;                   /* as for gpz_compress */
+  const void* x;                  /* device planes */
+  const void* y;
+  const void* z;                  /* NULL in 2D */
+  void* stream;                   /* device output, >= gpz_stream_bound(&p) bytes */
+  uint64_t cap;
+} gpz_batch_item;
+
+typedef struct gpz_batch_ditem {  /* one stream to decompress */
+  gpz_header h;                   /* from gpz_parse_header / gpz_read_header */
+  const void* stream;             /* device */
+  uint64_t bytes;
+  void* x;                        /* device outputs, h.n elements each */
+  void* y;
+  void* z;                        /* NULL in 2D */
+} gpz_batch_ditem;
+
+typedef struct gpz_batch_tuning {
+  uint32_t seg_groups;            /* 1024-point groups per queue segment: 1, 2, 4, 8 */
+  uint32_t waves_per_wg;          /* waves per persistent workgroup: 1, 2, 4, 8 */
+} gpz_batch_tuning;
+
+typedef struct gpz_batch_plan {   /* filled by the planners; pass back unchanged */
+  uint32_t count;
+  uint32_t kind;                  /* 0 compress, 1 decompress */
+  uint32_t seg_groups;
+  uint32_t waves_per_wg;
+  uint32_t cls_items[4];          /* classes: f32 2D, f32 3D, f64 2D, f64 3D */
+  uint32_t cls_first[4];
+  uint32_t cls_segs[4];
+  uint32_t cls_seg_first[4];
+  uint32_t grid[4];               /* persistent workgroups per class */
+  uint64_t total_groups;
+  uint64_t ws_bytes;
+} gpz_batch_plan;
+
+uint64_t gpz_batch_compress_workspace_bytes(const gpz_batch_item* items, uint32_t count);
+uint64_t gpz_batch_decompress_workspace_bytes(const gpz_batch_ditem* items, uint32_t count);
+/* tune NULL: one group per segment, one wave per workgroup. */
+int gpz_batch_plan_compress(const gpz_batch_item* items, uint32_t count,
+                            const gpz_batch_tuning* tune, void* d_ws, uint64_t ws_bytes,
+                            gpz_batch_plan* plan, gpz_stream_t stream);
+int gpz_batch_plan_decompress(const gpz_batch_ditem* items, uint32_t count,
+                              const gpz_batch_tuning* tune, void* d_ws, uint64_t ws_bytes,
+                              gpz_batch_plan* plan, gpz_stream_t stream);
+/* Asynchronous: launches the planned batch on `stream`. */
+int gpz_batch_run(const gpz_batch_plan* plan, void* d_ws, gpz_stream_t stream);
+/* Synchronises: out[i] = result of compressed snapshot i. */
+int gpz_batch_compress_results(const gpz_batch_plan* plan, const void* d_ws,
+                               gpz_result* out, gpz_stream_t stream);
+/* A hipGraphExec_t of gpz_batch_run(plan, d_ws), as an opaque pointer. */
+int gpz_batch_graph_create(const gpz_batch_plan* plan, void* d_ws, void** graph_exec);
+int gpz_batch_graph_launch(void* graph_exec, gpz_stream_t stream);
+int gpz_batch_graph_destroy(void* graph_exec);
+/* Autotuner: times every (seg_groups, waves_per_wg) pair over `reps` runs
+ * of the batch and returns the fastest; table (16 floats, may be NULL)
+ * receives ms per run at [4*log2(seg_groups) + log2(waves_per_wg)].
+ * Leaves the workspace planned with the last pair tried. */
+int gpz_batch_autotune_compress(const gpz_batch_item* items, uint32_t count, void* d_ws,
+                                uint64_t ws_bytes, gpz_stream_t stream, uint32_t reps,
+                                gpz_batch_tuning* best, float* table);
+int gpz_batch_autotune_decompress(const gpz_batch_ditem* items, uint32_t count, void* d_ws,
+                                  uint64_t ws_bytes, gpz_stream_t stream, uint32_t reps,
+                                  gpz_batch_tuning* best, float* table);
+
 #ifdef __cplusplus
 }
 #endif
