@@ -98,6 +98,97 @@ def cpu_baseline(x, y, z, spec, eb, seg, threads, sample_points, reps=2):
                 decompress_GBps=round(nbytes / td / 1e9, 4))
 
 
+def batch_leg(steps, warmup):
+    """Uneven mix: 256 snapshots (N log-uniform in [1e4, 4e6], 80 % 3D, 75 %
+    FP32, rel 10^U(-5,-3), seed 3).  Per-call = one gpz_compress (or
+    gpz_decompress) per snapshot, back to back on one stream with a single
+    synchronisation at the end (the most favourable per-call schedule);
+    batched = one hipGraph launch of the autotuned batch."""
+    import ctypes
+    import numpy as np
+    import torch
+    from gpz_amd import _native as N
+    from gpz_amd import api, batch, datagen
+    L = N.lib()
+    specs = datagen.batch_mix(256, seed=3)
+    snaps = []
+    in_bytes = 0
+    for sp in specs:
+        x, y, z = datagen.clustered(sp["n"], dim=sp["dim"], bits=sp["bits"],
+                                    clusters=sp["clusters"], seed=sp["seed"])
+        dev = [torch.from_numpy(a).cuda() for a in (x, y) + ((z,) if sp["dim"] == 3 else ())]
+        snaps.append((dev[0], dev[1], dev[2] if sp["dim"] == 3 else None, "rel", sp["e"], 0))
+        in_bytes += x.nbytes * sp["dim"]
+    torch.cuda.synchronize()
+    b = batch.BatchCompressor(snaps)
+    ctune, _ = b.autotune(reps=2)
+    b.run()
+    streams = b.streams()
+    out_bytes = sum(int(s.numel()) for s in streams)
+    d = batch.BatchDecompressor(streams)
+    dtune, _ = d.autotune(reps=2)
+
+    # per-call resources, allocated outside the timed region
+    s = api._stream()
+    calls = []
+    for p, (x, y, z) in zip(b.params, b.planes):
+        out = torch.empty(api.stream_bound(p), dtype=torch.uint8, device="cuda")
+        ws = torch.empty(api.workspace_bytes(p), dtype=torch.uint8, device="cuda")
+        calls.append((p, x, y, z, out, ws))
+    douts = []
+    for st, h in zip(streams, d.headers):
+        dt = torch.float64 if h.float_bits == 64 else torch.float32
+        douts.append([torch.empty(h.n, dtype=dt, device="cuda") for _ in range(h.dim)])
+
+    def per_call_compress():
+        for p, x, y, z, out, ws in calls:
+            L.gpz_compress(ctypes.byref(p), api._vp(x), api._vp(y), api._vp(z), api._vp(out),
+                           out.numel(), api._vp(ws), ws.numel(), s)
+
+    def per_call_decompress():
+        for st, h, o in zip(streams, d.headers, douts):
+            L.gpz_decompress(ctypes.byref(h), api._vp(st), st.numel(), api._vp(o[0]),
+                             api._vp(o[1]), api._vp(o[2]) if len(o) == 3 else None, s)
+
+    gc, gd = b.graph(), d.graph()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_pc = timed(per_call_compress)
+    t_pd = timed(per_call_decompress)
+    t_bc = timed(gc.launch)
+    t_bd = timed(gd.launch)
+    # the batched and per-call streams agree
+    for i in (0, 100, 255):
+        r = (N.Result)()
+        L.gpz_compress_result(api._vp(calls[i][5]), ctypes.byref(r), s)
+        assert torch.equal(calls[i][4][: r.stream_bytes], streams[i]), i
+    alg = in_bytes + out_bytes
+    return dict(snapshots=len(specs), points=int(sum(sp["n"] for sp in specs)),
+                in_bytes=in_bytes, stream_bytes=out_bytes,
+                compress_tuning=list(ctune), decompress_tuning=list(dtune),
+                per_call_compress_GBps=round(in_bytes / t_pc / 1e9, 2),
+                per_call_decompress_GBps=round(in_bytes / t_pd / 1e9, 2),
+                batched_compress_GBps=round(in_bytes / t_bc / 1e9, 2),
+                batched_decompress_GBps=round(in_bytes / t_bd / 1e9, 2),
+                compress_speedup=round(t_pc / t_bc, 2),
+                decompress_speedup=round(t_pd / t_bd, 2),
+                batched_compress_roofline_frac=round(alg / t_bc / 8e12, 4),
+                batched_decompress_roofline_frac=round(alg / t_bd / 8e12, 4),
+                ms=dict(per_call_compress=round(t_pc * 1e3, 3),
+                        per_call_decompress=round(t_pd * 1e3, 3),
+                        batched_compress=round(t_bc * 1e3, 3),
+                        batched_decompress=round(t_bd * 1e3, 3)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +197,8 @@ def main():
     ap.add_argument("--workload", default="large_f32_3d")
     ap.add_argument("--n", type=int, default=None, help="override point count")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-snapshot batched-vs-per-call leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-sample", type=int, default=1 << 62,
                     help="points of the snapshot the CPU baseline runs on")
@@ -239,6 +332,12 @@ def main():
     # correctness spot check of the timed output (decode of the last step)
     assert torch.isfinite(ox).all().item()
 
+    bleg = None
+    if not args.no_batch:
+        log("[bench] batch leg ...")
+        bleg = batch_leg(args.steps, args.warmup)
+        log(f"[bench] batch {json.dumps(bleg)}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -276,6 +375,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "batch": bleg,
     }
     print(json.dumps(line), flush=True)
 

@@ -1552,39 +1552,53 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
     }
     const Q bpre = inc_scan(bsum) - bsum;
     const uint32_t cpre = wave_scan(csum) - csum;
-    wave_sync();
+    if (U == nvalid) {
+      // every rank is its own run (all counts are 1): rank 16l+t has block
+      // id bid[t] + bpre, no run lookup needed
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const uint32_t i = 16u * lane + t;
-      if (i < U) {
-        s_bid[i] = static_cast<Q>(bid[t] + bpre);
-        const uint32_t st = cend[t] + cpre - cnt[t];
-        if (st < kGroup) s_start[st] = 1;
+      for (int k = 0; k < 16; ++k) {
+        const Q rl = static_cast<Q>(rel[k]);
+        Q bx, by, bz;
+        split(static_cast<Q>(bid[k] + bpre), &bx, &by, &bz);
+        ox[k] = dequant<T>((((bx + (rl & m)) << 1) | 1u), eb, mm[0]);
+        oy[k] = dequant<T>((((by + (shr(rl, sh) & m)) << 1) | 1u), eb, mm[1]);
+        if (DIM == 3) oz[k] = dequant<T>((((bz + shr(rl, 2u * sh)) << 1) | 1u), eb, mm[2]);
       }
-    }
-    wave_sync();
-    // ---- rank -> run index by a prefix count of run starts
-    const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
-    const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
-    uint32_t rs = 0;
-    uint32_t runof[16];
+    } else {
+      wave_sync();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      rs += (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
-      runof[k] = rs;
-    }
-    const uint32_t rpre = wave_scan(rs) - rs;
+      for (int t = 0; t < 16; ++t) {
+        const uint32_t i = 16u * lane + t;
+        if (i < U) {
+          s_bid[i] = static_cast<Q>(bid[t] + bpre);
+          const uint32_t st = cend[t] + cpre - cnt[t];
+          if (st < kGroup) s_start[st] = 1;
+        }
+      }
+      wave_sync();
+      // ---- rank -> run index by a prefix count of run starts
+      const uint4 fv = reinterpret_cast<const uint4*>(s_start)[lane];
+      const uint32_t fw[4] = {fv.x, fv.y, fv.z, fv.w};
+      uint32_t rs = 0;
+      uint32_t runof[16];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      uint32_t run = runof[k] + rpre;
-      run = run ? run - 1u : 0u;
-      if (run >= U) run = U - 1u;
-      const Q rl = static_cast<Q>(rel[k]);
-      Q bx, by, bz;
-      split(s_bid[run], &bx, &by, &bz);
-      ox[k] = dequant<T>((((bx + (rl & m)) << 1) | 1u), eb, mm[0]);
-      oy[k] = dequant<T>((((by + (shr(rl, sh) & m)) << 1) | 1u), eb, mm[1]);
-      if (DIM == 3) oz[k] = dequant<T>((((bz + shr(rl, 2u * sh)) << 1) | 1u), eb, mm[2]);
+      for (int k = 0; k < 16; ++k) {
+        rs += (fw[k >> 2] >> (8 * (k & 3))) & 0xFFu;
+        runof[k] = rs;
+      }
+      const uint32_t rpre = wave_scan(rs) - rs;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        uint32_t run = runof[k] + rpre;
+        run = run ? run - 1u : 0u;
+        if (run >= U) run = U - 1u;
+        const Q rl = static_cast<Q>(rel[k]);
+        Q bx, by, bz;
+        split(s_bid[run], &bx, &by, &bz);
+        ox[k] = dequant<T>((((bx + (rl & m)) << 1) | 1u), eb, mm[0]);
+        oy[k] = dequant<T>((((by + (shr(rl, sh) & m)) << 1) | 1u), eb, mm[1]);
+        if (DIM == 3) oz[k] = dequant<T>((((bz + shr(rl, 2u * sh)) << 1) | 1u), eb, mm[2]);
+      }
     }
   }
 
@@ -1679,14 +1693,43 @@ __device__ __forceinline__ uint32_t wave_in_block() {
       __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)));
 }
 
+// One wave per sample window (4 points per lane), four windows per
+// workgroup; same window extrema as seg_sample_window.
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void batch_seg_sample_kernel(const BatchItem* items,
                                                                const uint32_t* cls) {
   const BatchItem& it = items[cls[blockIdx.y]];
   if (!it.sa.sampled) return;
-  seg_sample_window<T, DIM>(static_cast<const T*>(it.e.x), static_cast<const T*>(it.e.y),
-                            static_cast<const T*>(it.e.z), it.e.n, it.stride,
-                            static_cast<T*>(it.win), blockIdx.x);
+  const uint32_t lane = threadIdx.x & 63u, b = blockIdx.x * 4 + wave_in_block();
+  T* win = static_cast<T*>(it.win);
+  const uint64_t st = static_cast<uint64_t>(b) * it.stride;
+  if (st + kSampleLen > it.e.n) {
+    if (lane < 6) win[b * 6 + lane] = T(0);
+    return;
+  }
+  const T* P3[3] = {static_cast<const T*>(it.e.x), static_cast<const T*>(it.e.y),
+                    static_cast<const T*>(it.e.z)};
+  T r = T(0);
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    T lo = Prec<T>::lo(), hi = -Prec<T>::lo();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const T v = P3[a][st + 64u * k + lane];
+      lo = dmin(lo, v);
+      hi = dmax(hi, v);
+    }
+    if constexpr (sizeof(T) == 4) {
+      lo = wave_ext<false>(lo);
+      hi = wave_ext<true>(hi);
+    } else {
+      lo = wave_min(lo);
+      hi = wave_max(hi);
+    }
+    if (lane == 2u * a) r = lo;
+    if (lane == 2u * a + 1u) r = hi;
+  }
+  if (lane < 6) win[b * 6 + lane] = r;  // 2D: z extrema stay 0
 }
 
 template <typename T, int DIM>
@@ -2064,6 +2107,18 @@ int upload(void* d_ws, const std::vector<uint8_t>& host, hipStream_t s) {
   return GPZ_OK;
 }
 
+// Timer names per class (f32 2D, f32 3D, f64 2D, f64 3D) and stage.
+const char* batch_timer(int stage, int k) {
+  static const char* names[5][kClasses] = {
+      {"batch_seg/f32_2d", "batch_seg/f32_3d", "batch_seg/f64_2d", "batch_seg/f64_3d"},
+      {"batch_encode/f32_2d", "batch_encode/f32_3d", "batch_encode/f64_2d", "batch_encode/f64_3d"},
+      {"batch_scan/f32_2d", "batch_scan/f32_3d", "batch_scan/f64_2d", "batch_scan/f64_3d"},
+      {"batch_compact/f32_2d", "batch_compact/f32_3d", "batch_compact/f64_2d",
+       "batch_compact/f64_3d"},
+      {"batch_decode/f32_2d", "batch_decode/f32_3d", "batch_decode/f64_2d", "batch_decode/f64_3d"}};
+  return names[stage][k];
+}
+
 template <typename T, int DIM>
 void batch_compress_class(const gpz_batch_plan* plan, int k, uint8_t* ws, hipStream_t s,
                           bool timed) {
@@ -2075,23 +2130,23 @@ void batch_compress_class(const gpz_batch_plan* plan, int k, uint8_t* ws, hipStr
   const Segment* segs = reinterpret_cast<const Segment*>(ws + L.segs) + plan->cls_seg_first[k];
   const uint32_t W = plan->waves_per_wg;
   {
-    LaunchTimer lt(timed ? "batch_seg" : nullptr, s);
-    hipLaunchKernelGGL((batch_seg_sample_kernel<T, DIM>), dim3(kSampleBlocks, ni), dim3(256), 0,
-                       s, items, cls);
+    LaunchTimer lt(timed ? batch_timer(0, k) : nullptr, s);
+    hipLaunchKernelGGL((batch_seg_sample_kernel<T, DIM>), dim3(kSampleBlocks / 4, ni), dim3(256),
+                       0, s, items, cls);
     hipLaunchKernelGGL((batch_seg_decide_kernel<T, DIM>), dim3(ni), dim3(256), 0, s, items, cls);
   }
   {
-    LaunchTimer lt(timed ? "batch_encode" : nullptr, s);
+    LaunchTimer lt(timed ? batch_timer(1, k) : nullptr, s);
     const size_t lds = static_cast<size_t>(W) * kLdsBytes;
     hipLaunchKernelGGL((batch_encode_kernel<T, DIM>), dim3(plan->grid[k]), dim3(64 * W), lds, s,
                        items, segs, ns);
   }
   {
-    LaunchTimer lt(timed ? "batch_scan" : nullptr, s);
+    LaunchTimer lt(timed ? batch_timer(2, k) : nullptr, s);
     hipLaunchKernelGGL((batch_scan_kernel<T>), dim3(ni), dim3(256), 0, s, items, cls);
   }
   {
-    LaunchTimer lt(timed ? "batch_compact" : nullptr, s);
+    LaunchTimer lt(timed ? batch_timer(3, k) : nullptr, s);
     const uint32_t need = (ns + 3) / 4, fill = 4096;
     hipLaunchKernelGGL((batch_compact_kernel<T>), dim3(need < fill ? need : fill), dim3(256), 0,
                        s, items, segs, ns);
@@ -2107,7 +2162,7 @@ void batch_decompress_class(const gpz_batch_plan* plan, int k, uint8_t* ws, hipS
   const DecodeArgs* items = reinterpret_cast<const DecodeArgs*>(ws + L.items);
   const Segment* segs = reinterpret_cast<const Segment*>(ws + L.segs) + plan->cls_seg_first[k];
   const uint32_t W = plan->waves_per_wg;
-  LaunchTimer lt(timed ? "batch_decode" : nullptr, s);
+  LaunchTimer lt(timed ? batch_timer(4, k) : nullptr, s);
   const size_t lds = static_cast<size_t>(W) * dec_lds_bytes<T>();
   hipLaunchKernelGGL((batch_decode_kernel<T, DIM>), dim3(plan->grid[k]), dim3(64 * W), lds, s,
                      items, segs, ns);
@@ -2133,18 +2188,69 @@ void plan_grids(gpz_batch_plan* plan) {
   plan->grid[3] = plan_grid<double, 3>(kind, W, plan->cls_segs[3]);
 }
 
+// Per device: three auxiliary streams (default priority) and events, so
+// that the classes of a batch run concurrently; under capture the fork and
+// join become parallel graph branches.
+struct ForkStreams {
+  hipStream_t aux[kClasses - 1] = {};
+  hipEvent_t fork = nullptr, join[kClasses - 1] = {};
+  bool ok = false;
+};
+ForkStreams* fork_streams() {
+  static std::mutex mu;
+  static std::map<int, ForkStreams> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  ForkStreams& f = cache[dev];
+  if (!f.ok) {
+    bool ok = hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < kClasses - 1 && ok; ++i)
+      ok = hipStreamCreateWithFlags(&f.aux[i], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&f.join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) return nullptr;
+    f.ok = true;
+  }
+  return &f;
+}
+
 int batch_run_impl(const gpz_batch_plan* plan, void* d_ws, hipStream_t s, bool timed) {
   uint8_t* ws = static_cast<uint8_t*>(d_ws);
-  if (plan->kind == 0) {
-    batch_compress_class<float, 2>(plan, 0, ws, s, timed);
-    batch_compress_class<float, 3>(plan, 1, ws, s, timed);
-    batch_compress_class<double, 2>(plan, 2, ws, s, timed);
-    batch_compress_class<double, 3>(plan, 3, ws, s, timed);
-  } else {
-    batch_decompress_class<float, 2>(plan, 0, ws, s, timed);
-    batch_decompress_class<float, 3>(plan, 1, ws, s, timed);
-    batch_decompress_class<double, 2>(plan, 2, ws, s, timed);
-    batch_decompress_class<double, 3>(plan, 3, ws, s, timed);
+  ForkStreams* fs = fork_streams();
+  if (!fs) return GPZ_ERR_HIP;
+  // classes with work, largest first on the caller's stream
+  int order[kClasses], n = 0;
+  for (int k = 0; k < kClasses; ++k)
+    if (plan->cls_items[k]) order[n++] = k;
+  for (int a = 0; a < n; ++a)
+    for (int b = a + 1; b < n; ++b)
+      if (plan->cls_segs[order[b]] * static_cast<uint64_t>(plan->seg_groups) >
+          plan->cls_segs[order[a]] * static_cast<uint64_t>(plan->seg_groups)) {
+        const int t = order[a]; order[a] = order[b]; order[b] = t;
+      }
+  if (n > 1 && (hipEventRecord(fs->fork, s) != hipSuccess)) return GPZ_ERR_HIP;
+  for (int i = 0; i < n; ++i) {
+    hipStream_t cs = i == 0 ? s : fs->aux[i - 1];
+    if (i > 0 && hipStreamWaitEvent(cs, fs->fork, 0) != hipSuccess) return GPZ_ERR_HIP;
+    const int k = order[i];
+    if (plan->kind == 0) {
+      switch (k) {
+        case 0: batch_compress_class<float, 2>(plan, 0, ws, cs, timed); break;
+        case 1: batch_compress_class<float, 3>(plan, 1, ws, cs, timed); break;
+        case 2: batch_compress_class<double, 2>(plan, 2, ws, cs, timed); break;
+        default: batch_compress_class<double, 3>(plan, 3, ws, cs, timed); break;
+      }
+    } else {
+      switch (k) {
+        case 0: batch_decompress_class<float, 2>(plan, 0, ws, cs, timed); break;
+        case 1: batch_decompress_class<float, 3>(plan, 1, ws, cs, timed); break;
+        case 2: batch_decompress_class<double, 2>(plan, 2, ws, cs, timed); break;
+        default: batch_decompress_class<double, 3>(plan, 3, ws, cs, timed); break;
+      }
+    }
+    if (i > 0 && (hipEventRecord(fs->join[i - 1], cs) != hipSuccess ||
+                  hipStreamWaitEvent(s, fs->join[i - 1], 0) != hipSuccess))
+      return GPZ_ERR_HIP;
   }
   return hipGetLastError() == hipSuccess ? GPZ_OK : GPZ_ERR_HIP;
 }

@@ -190,3 +190,18 @@ def test_decode_stays_inside_outputs(oracle, bits, dim, n):
             np.testing.assert_array_equal(seg[:n], d_o[a])
         else:
             assert (seg[:n] == -12345.0).all()
+
+
+@pytest.mark.parametrize("bits,dim", [(32, 3), (64, 2)])
+def test_all_distinct_blocks(oracle, bits, dim):
+    """Every point in its own block (U = points of the group): the
+    decoder's rank-is-run path, full and partial groups."""
+    rng = np.random.default_rng(5)
+    n = 4 * 1024 + 333
+    dt = np.float64 if bits == 64 else np.float32
+    x, y, z = (rng.uniform(0, 1000, n).astype(dt) for _ in range(3))
+    zz = z if dim == 3 else None
+    s_o, r, d = _parity(oracle, x, y, zz, "abs", 1e-3, seg_in=1)
+    from gpz_amd import api
+    h = api.parse_header(s_o)
+    assert h.groups == 5
