@@ -152,7 +152,10 @@ class Server:
                                 float(np.float32(self.e)), "rel", 0)
             req["params"] = p
             req["out_bytes"] = [api.stream_bound(p)]
-            need = req["out_bytes"][0] + api.workspace_bytes(p) + 4 * ALIGN
+            one = (N.BatchItem * 1)()
+            one[0].p = p
+            need = (req["out_bytes"][0] + 4 * ALIGN
+                    + int(self.L.gpz_batch_compress_workspace_bytes(one, 1)))
         else:
             req["out_bytes"] = [4 * req["n"]] * 3
             need = 12 * req["n"] + 4 * ALIGN

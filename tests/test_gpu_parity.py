@@ -131,6 +131,20 @@ def test_negative_and_offset_coordinates(oracle):
     _parity(oracle, x - 1e4, -y, z * 0.001 + 5e3, "rel", 1e-4)
 
 
+@pytest.mark.parametrize("bits,dim,n", [
+    (32, 3, 3 * 1048576 + 5000),
+    (32, 3, 8 * 1048576 + 777),
+    (64, 3, 4 * 1048576 + 1),
+    (32, 2, 17 * 1048576 - 3),
+])
+def test_multi_million_points(oracle, bits, dim, n):
+    """Millions of points (thousands of groups, ragged last group): payload
+    offsets accumulate over many groups."""
+    from gpz_amd import datagen
+    x, y, z = datagen.clustered(n, dim=dim, bits=bits, clusters=256, seed=n % 1000)
+    _parity(oracle, x, y, z, "rel", 1e-4)
+
+
 def _group_sorted_bound(orig, dec, eb):
     """Per group and axis, sorted original vs sorted decoded within the bound
     (holds whenever every point is within the bound)."""
