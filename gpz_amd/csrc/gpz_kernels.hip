@@ -906,21 +906,23 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   // low 10 bits of one word and the values stay put, indexed by position.
   const uint32_t sortbits = bidbits + (last ? 1u : 0u);
   const ScanLane sl = scan_lane(lane);
-  if (sortbits <= 22) {
+  if (sortbits == 0) {
+    // one block id: the blocked order is already sorted
+  } else if (sortbits <= 22) {
 #pragma unroll
     for (int k = 0; k < 16; ++k) key[k] = (key[k] << 10) | (16u * lane + k);
     for (uint32_t shift = 10; shift < 10 + sortbits; shift += 4)
       radix_pass<false>(key, val, shift, lane, sl, lds);
     wave_sync();
+    // values by position in 17-word rows: the stores (lane l writes row l)
+    // and a gather of nearly sorted positions are free of bank conflicts
     uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kGatherB);
 #pragma unroll
-    for (int q4 = 0; q4 < 4; ++q4)
-      reinterpret_cast<uint4*>(s_val)[4 * lane + q4] =
-          make_uint4(val[4 * q4], val[4 * q4 + 1], val[4 * q4 + 2], val[4 * q4 + 3]);
+    for (int k = 0; k < 16; ++k) s_val[17 * lane + k] = val[k];
     wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      val[k] = s_val[key[k] & 1023u];
+      val[k] = s_val[pad_word(key[k] & 1023u)];
       key[k] >>= 10;
     }
     // invalid ranks keep all-ones block ids (never compared below)
@@ -949,7 +951,10 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   const uint32_t fincl = wave_scan(nf);
   const uint32_t U = lane63(fincl);
   const bool small = U <= 64;
-  {
+  // Every valid rank starts its own run: deltas and counts (all 1) stay in
+  // registers and the run tables are not built.
+  const bool distinct = !small && U == nvalid;
+  if (!distinct) {
     // deltas past U read 0 and starts past U read nvalid, so that counts
     // past U come out 0
     const uint32_t nv2 = nvalid | (nvalid << 16);
@@ -1023,7 +1028,14 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     if (lane + 64 < nw) sl32[lane + 64] = st32[lane + 64];
   } else {
     uint32_t del[16], cnt[16];
-    {
+    if (distinct) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const bool v = kFull || 16u * lane + k < nvalid;
+        del[k] = v ? key[k] - (k ? key[k - 1] : prev0) : 0u;
+        cnt[k] = v ? 1u : 0u;
+      }
+    } else {
       const uint4* d4 = reinterpret_cast<const uint4*>(s_rdelta) + 4 * lane;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
