@@ -595,6 +595,13 @@ static_assert(kRunStartB + 2 * 1040 <= 4 * kRunDump &&
 __device__ __forceinline__ uint32_t pad_word(uint32_t p) {
   return p + (p >> 4);
 }
+// Byte offset of pad_word(p), 4 * (p + p/16), in two VALU operations (the
+// compiler's own form of the expression takes three).
+__device__ __forceinline__ uint32_t pad_byte(uint32_t p) {
+  uint32_t a;
+  asm("v_add_lshl_u32 %0, %1, %2, 2" : "=v"(a) : "v"(p), "v"(p >> 4));
+  return a;
+}
 
 // Quantised code of one coordinate: sat_u32 / sat_u64 of fma(x, 1/(2eb),
 // -min/(2eb)) (src/cudaWarpCompress.cu:179-184,
@@ -693,7 +700,7 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
   if (kPair) wave_sync();
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    s_key[pad_word(rnk[k])] = key[k];
+    *reinterpret_cast<uint32_t*>(lds + kLdsKeyB + pad_byte(rnk[k])) = key[k];
     if (kPair) s_val[rnk[k]] = val[k];
   }
   wave_sync();
@@ -922,7 +929,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
     wave_sync();
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      val[k] = s_val[pad_word(key[k] & 1023u)];
+      val[k] = *reinterpret_cast<const uint32_t*>(lds + kGatherB + pad_byte(key[k] & 1023u));
       key[k] >>= 10;
     }
     // invalid ranks keep all-ones block ids (never compared below)
