@@ -789,10 +789,20 @@ __device__ __forceinline__ void quantise(const T (&px)[16], const T (&py)[16],
                    lz = static_cast<uint32_t>(qz);
     uint32_t b, r;
     if (kFast) {
-      b = sx + __umul24(sy, nbx32);
-      if (DIM == 3) b += __umul24(sz, nbxy32);
-      r = (lx & m) | ((ly & m) << sh);
-      if (DIM == 3) r |= (lz & m) << (2u * sh);
+      // b = sx + sy*nbx + sz*nbxy with 24-bit multiply-adds, and
+      // r = ((lz&m) << 2sh) | ((ly&m) << sh) | (lx&m) as shift-or chains
+      // (five operations each; the compiler's own forms take six)
+      if (DIM == 3) {
+        uint32_t t;
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(t) : "v"(sz), "s"(nbxy32), "v"(sx));
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(b) : "v"(sy), "s"(nbx32), "v"(t));
+        uint32_t u;
+        asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(u) : "v"(lz & m), "s"(sh), "v"(ly & m));
+        asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(u), "s"(sh), "v"(lx & m));
+      } else {
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(b) : "v"(sy), "s"(nbx32), "v"(sx));
+        asm("v_lshl_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(ly & m), "s"(sh), "v"(lx & m));
+      }
     } else {
       b = sx + sy * nbx32;
       if (DIM == 3) b += sz * nbxy32;
