@@ -1681,7 +1681,7 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
 }
 
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 3)))
 void decode_kernel(DecodeArgs a) {
   __shared__ __align__(16) uint8_t lds[dec_lds_bytes<T>()];
   decode_group<T, DIM>(a, blockIdx.x, threadIdx.x, lds);
