@@ -833,6 +833,87 @@ __device__ __forceinline__ void quantise(const T (&px)[16], const T (&py)[16],
   *ror_out = ror;
 }
 
+// The same stage for FP64, one axis at a time: the coordinates are read
+// again (from L2: the group was read for its extrema just before) and each
+// axis's codes are folded into the block ids and relative positions before
+// the next axis is read, so that 32 rather than 96 registers hold doubles.
+// Sums and products are taken in the same 32-bit wrap-around arithmetic as
+// quantise(), whose result they equal.
+template <typename T, int DIM, bool kFull, bool kFast>
+__device__ __forceinline__ void quantise_axes(const T* X, const T* Y, const T* Z,
+                                              T inv, const T (&nmi)[3],
+                                              uint32_t seg, uint32_t nbx32, uint32_t nbxy32,
+                                              uint32_t jbase, uint32_t nvalid,
+                                              uint32_t (&key)[16], uint32_t (&val)[16],
+                                              uint32_t* bor_out, uint32_t* ror_out) {
+  const uint32_t sh = static_cast<uint32_t>(__builtin_ctz(seg));
+  const uint32_t m = seg - 1u;
+  const bool pow2 = (seg & m) == 0u;
+  const uint32_t zsh = (2u * sh) & 31u;
+  const uint32_t mz = 2u * sh >= 32 ? 0u : m;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const T* P = d == 0 ? X : d == 1 ? Y : Z;
+    uint32_t jb = jbase;
+    asm volatile("" : "+v"(jb));  // a fresh read, not the registers of the first
+    T v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      uint32_t j = jb + 32u * k;
+      if (!kFull) j = j < nvalid ? j : nvalid - 1u;
+      v[k] = P[j];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const auto q = quant_q<T>(v[k], inv, nmi[d]);
+      const uint32_t s = static_cast<uint32_t>(q >> sh);
+      const uint32_t l = static_cast<uint32_t>(q);
+      if (d == 0) {
+        key[k] = s;
+        val[k] = l & m;
+      } else if (d == 1) {
+        if (kFast) {
+          asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(key[k]) : "v"(s), "s"(nbx32), "v"(key[k]));
+        } else {
+          key[k] += s * nbx32;
+        }
+        val[k] = (kFast || pow2) ? val[k] | ((l & m) << sh) : val[k] + (l & m) * seg;
+      } else {
+        if (kFast) {
+          asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(key[k]) : "v"(s), "s"(nbxy32), "v"(key[k]));
+          val[k] |= (l & m) << (2u * sh);
+        } else {
+          key[k] += s * nbxy32;
+          val[k] = pow2 ? val[k] | ((l & mz) << zsh) : val[k] + (l & m) * seg * seg;
+        }
+      }
+    }
+  }
+  uint32_t bor = 0, ror = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    uint32_t b = key[k], r = val[k];
+    if (!kFast) {
+      b &= 0x7FFFFFFFu;
+      r &= 0x7FFFFFFFu;
+    }
+    if (kFull) {
+      key[k] = b;
+      val[k] = r;
+      bor |= b;
+      ror |= r;
+    } else {
+      const bool v = jbase + 32u * k < nvalid;
+      key[k] = v ? b : 0xFFFFFFFFu;
+      val[k] = v ? r : 0xFFFFFFFFu;
+      bor |= v ? b : 1u;
+      ror |= v ? r : 1u;
+    }
+  }
+  *bor_out = bor;
+  *ror_out = ror;
+}
+
 template <typename T, int DIM, bool kFull>
 __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
                                              uint32_t lane, uint8_t* lds) {
@@ -909,12 +990,24 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
            s0 + s1 * bx + s2 * bxy <= 0x7FFFFFFFull;
   }
   uint32_t key[16], val[16], bor, ror;
-  if (fast)
-    quantise<T, DIM, kFull, true>(px, py, pz, inv, nmi, seg, nbx32, nbxy32, jbase,
-                                  nvalid, key, val, &bor, &ror);
-  else
-    quantise<T, DIM, kFull, false>(px, py, pz, inv, nmi, seg, nbx32, nbxy32, jbase,
-                                   nvalid, key, val, &bor, &ror);
+  if constexpr (sizeof(T) == 8) {
+    const T* X = static_cast<const T*>(a.x) + base;
+    const T* Y = static_cast<const T*>(a.y) + base;
+    const T* Z = DIM == 3 ? static_cast<const T*>(a.z) + base : nullptr;
+    if (fast)
+      quantise_axes<T, DIM, kFull, true>(X, Y, Z, inv, nmi, seg, nbx32, nbxy32, jbase,
+                                         nvalid, key, val, &bor, &ror);
+    else
+      quantise_axes<T, DIM, kFull, false>(X, Y, Z, inv, nmi, seg, nbx32, nbxy32, jbase,
+                                          nvalid, key, val, &bor, &ror);
+  } else {
+    if (fast)
+      quantise<T, DIM, kFull, true>(px, py, pz, inv, nmi, seg, nbx32, nbxy32, jbase,
+                                    nvalid, key, val, &bor, &ror);
+    else
+      quantise<T, DIM, kFull, false>(px, py, pz, inv, nmi, seg, nbx32, nbxy32, jbase,
+                                     nvalid, key, val, &bor, &ror);
+  }
   const uint32_t bidbits = eff_bits(wave_or(bor));
   const uint32_t rb = eff_bits(wave_or(ror));
 
@@ -1161,7 +1254,7 @@ __device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t c,
 }
 
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
 void encode_kernel(EncodeArgs a) {
   __shared__ __align__(16) uint8_t lds[kLdsBytes];
   if (a.info->status != GPZ_OK) return;
