@@ -1455,6 +1455,51 @@ template <typename T> constexpr uint32_t dec_lds_bytes() {
   return kGroup * sizeof(typename Prec<T>::Q) + kGroup;
 }
 
+// Values of reads issued under separate branches, pinned where all of them
+// have started: the compiler cannot sink their first use (and with it a
+// wait for the read) into each branch.
+template <int N>
+__device__ __forceinline__ void pin_reads(uint32_t (&v)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) asm volatile("" : "+v"(v[i]));
+}
+
+// Lane l's bytes 2l and 2l+1 (b0 | b1 << 8) of the MSB-first planes
+// b0 + j, j < 16, of a section with nb planes of B bytes: plane b sits at
+// (nb-1-b)*B; planes at or past nb read 0.  Each read starts under its own
+// uniform `b < nb` and nothing uses it before the last one has started, so
+// the reads are in flight together (a use inside the branch makes every
+// plane wait for the one before).  With B even the two bytes are one
+// aligned 16-bit read; lanes past the plane (2l >= B) re-read byte 0.
+__device__ __forceinline__ void read_plane_pairs(const uint8_t* sec, uint32_t nb, uint32_t B,
+                                                 uint32_t lane, int b0, uint32_t (&h)[16]) {
+  const uint32_t o0 = 2u * lane < B ? 2u * lane : 0u;
+  const uint32_t o1 = 2u * lane + 1u < B ? 2u * lane + 1u : 0u;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) h[j] = 0;
+  if ((B & 1u) == 0u) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (b0 + j < static_cast<int>(nb))
+        h[j] = *reinterpret_cast<const uint16_t*>(sec + (nb - 1u - (b0 + j)) * B + o0);
+    pin_reads(h);
+  } else {
+    uint32_t hi[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) hi[j] = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+      if (b0 + j < static_cast<int>(nb)) {
+        h[j] = sec[(nb - 1u - (b0 + j)) * B + o0];
+        hi[j] = sec[(nb - 1u - (b0 + j)) * B + o1];
+      }
+    pin_reads(h);
+    pin_reads(hi);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) h[j] |= hi[j] << 8;
+  }
+}
+
 template <typename T, int DIM>
 __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
                                              uint32_t lane, uint8_t* lds) {
@@ -1509,15 +1554,21 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
         rel[i + 8] = r[i] >> 16;
       }
     } else {
+      // reads under `b < rb`, first used after the last one (see
+      // read_plane_pairs)
 #pragma unroll
       for (int k = 0; k < 16; ++k) rel[k] = 0;
+      uint32_t hw[16];
 #pragma unroll
-      for (int b = 0; b < 32; ++b) {
-        if (b < static_cast<int>(rb)) {
-          const uint32_t w = plane16(b);
-          rel[b & 15] |= b < 16 ? w : (w << 16);
-        }
-      }
+      for (int k = 0; k < 16; ++k) hw[k] = 0;
+#pragma unroll
+      for (int b = 0; b < 16; ++b) rel[b] = plane16(b);
+#pragma unroll
+      for (int b = 16; b < 32; ++b)
+        if (b < static_cast<int>(rb)) hw[b - 16] = plane16(b);
+      pin_reads(hw);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) rel[k] |= hw[k] << 16;
       transpose16x2(rel);
     }
   }
@@ -1557,12 +1608,18 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
     const bool valid = u < U;
     const uint32_t boff = u < 8u * B ? (u >> 3) : 0u;
     const uint32_t bsh = 7u - (u & 7u);
+    // reads under `b < db` / `b < cb`, first used after the last one (see
+    // read_plane_pairs)
     uint32_t pd[16], pc[16];
 #pragma unroll
+    for (int b = 0; b < 16; ++b) { pd[b] = 0; pc[b] = 0; }
+#pragma unroll
     for (int b = 0; b < 16; ++b) {
-      pd[b] = b < static_cast<int>(db) ? pay[(db - 1u - b) * B + boff] : 0u;
-      pc[b] = b < static_cast<int>(cb) ? pay[D + (cb - 1u - b) * B + boff] : 0u;
+      if (b < static_cast<int>(db)) pd[b] = pay[(db - 1u - b) * B + boff];
+      if (b < static_cast<int>(cb)) pc[b] = pay[D + (cb - 1u - b) * B + boff];
     }
+    pin_reads(pd);
+    pin_reads(pc);
     uint32_t dv = 0, cv = 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
@@ -1606,19 +1663,13 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
     // (b0 | b1 << 16) through the encoder's transpose in reverse; slot s
     // then holds run 16l + (s^7).  Bytes past the plane (2l >= B) only feed
     // runs >= U; they re-read byte 0 to stay inside the stream.
-    const uint32_t o0 = 2u * lane < B ? 2u * lane : 0u;
-    const uint32_t o1 = 2u * lane + 1u < B ? 2u * lane + 1u : 0u;
-    auto plane2 = [&](uint32_t off) -> uint32_t {
-      return static_cast<uint32_t>(pay[off + o0]) | (static_cast<uint32_t>(pay[off + o1]) << 16);
-    };
     uint32_t del[16], cnt[16];
     if (db <= 16) {
       uint32_t w[16];
-#pragma unroll
-      for (int b = 0; b < 16; ++b) w[b] = b < static_cast<int>(db) ? plane2((db - 1u - b) * B) : 0u;
+      read_plane_pairs(pay, db, B, lane, 0, w);
       uint32_t r[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(w[i + 8], w[i], 0x06020400u);
+      for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(w[i + 8], w[i], 0x05010400u);
       transpose16_bytes(r);
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
@@ -1626,28 +1677,21 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
         del[15 - i] = r[i] >> 16;
       }
     } else {
-      uint32_t t16[16];
+      uint32_t t16[16], hw[16];
+      read_plane_pairs(pay, db, B, lane, 0, t16);
+      read_plane_pairs(pay, db, B, lane, 16, hw);
 #pragma unroll
-      for (int t = 0; t < 16; ++t) t16[t] = 0;
-#pragma unroll
-      for (int b = 0; b < 32; ++b) {
-        if (b < static_cast<int>(db)) {
-          const uint32_t x = plane2((db - 1u - b) * B);
-          const uint32_t w = (x & 0xFFu) | ((x >> 8) & 0xFF00u);
-          t16[b & 15] |= b < 16 ? w : (w << 16);
-        }
-      }
+      for (int t = 0; t < 16; ++t) t16[t] |= hw[t] << 16;
       transpose16x2(t16);
 #pragma unroll
       for (int t = 0; t < 16; ++t) del[t ^ 7] = t16[t];
     }
     {
       uint32_t w[16];
-#pragma unroll
-      for (int b = 0; b < 16; ++b) w[b] = b < static_cast<int>(cb) ? plane2(D + (cb - 1u - b) * B) : 0u;
+      read_plane_pairs(pay + D, cb, B, lane, 0, w);
       uint32_t r[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(w[i + 8], w[i], 0x06020400u);
+      for (int i = 0; i < 8; ++i) r[i] = __builtin_amdgcn_perm(w[i + 8], w[i], 0x05010400u);
       transpose16_bytes(r);
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
