@@ -1454,6 +1454,12 @@ __device__ __forceinline__ T dequant(Q q2, T eb, T mn) {
 template <typename T> constexpr uint32_t dec_lds_bytes() {
   return kGroup * sizeof(typename Prec<T>::Q) + kGroup;
 }
+// The small-group delta/count bytes (256) sit after the 64-run table.
+template <typename T> constexpr uint32_t kDecSmallDcB() {
+  return 4u * 64u * sizeof(typename Prec<T>::Q);
+}
+static_assert(4u * 64u * 4u + 256u <= 1024u * 4u && 4u * 64u * 8u + 256u <= 1024u * 8u,
+              "small-group delta/count bytes stay below the run-start map");
 
 // Values of reads issued under separate branches, pinned where all of them
 // have started: the compiler cannot sink their first use (and with it a
@@ -1532,6 +1538,17 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
   const uint8_t* pay = a.stream + pay0;
   const uint32_t B = (U + 7u) >> 3;
 
+  // Groups of <= 64 runs with <= 16 delta bits hold one run per lane.
+  // Their delta and count sections (D + C <= 216 bytes) are read here as
+  // one word per lane, before the relative-position planes, so that both
+  // reads are in flight at once; the bytes are picked out through LDS
+  // below.  Lanes past the sections re-read word 0 (C >= 4: every group
+  // has a count plane).
+  const bool small = U <= 64 && db <= 16;
+  uint32_t dcw = 0;
+  if (small)
+    dcw = *reinterpret_cast<const uint32_t*>(pay + (4u * lane < D + C ? 4u * lane : 0u));
+
   // ---- relative positions of ranks 16l..16l+15 (LSB-first planes, one
   // 16-bit word per lane and plane)
   uint32_t rel[16];
@@ -1601,25 +1618,22 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
   };
 
   T ox[16], oy[16], oz[16];
-  if (U <= 64 && db <= 16) {
+  if (small) {
     // ---- one run per lane.  Plane bytes past the plane (u >= 8B) only
-    // feed runs >= U; they re-read byte 0 to stay inside the stream.
+    // feed runs >= U; they re-read byte 0 of the plane.
     const uint32_t u = lane;
     const bool valid = u < U;
     const uint32_t boff = u < 8u * B ? (u >> 3) : 0u;
     const uint32_t bsh = 7u - (u & 7u);
-    // reads under `b < db` / `b < cb`, first used after the last one (see
-    // read_plane_pairs)
+    uint8_t* s_dc = lds + kDecSmallDcB<T>();
+    reinterpret_cast<uint32_t*>(s_dc)[lane] = dcw;
+    wave_sync();
     uint32_t pd[16], pc[16];
 #pragma unroll
-    for (int b = 0; b < 16; ++b) { pd[b] = 0; pc[b] = 0; }
-#pragma unroll
     for (int b = 0; b < 16; ++b) {
-      if (b < static_cast<int>(db)) pd[b] = pay[(db - 1u - b) * B + boff];
-      if (b < static_cast<int>(cb)) pc[b] = pay[D + (cb - 1u - b) * B + boff];
+      pd[b] = b < static_cast<int>(db) ? s_dc[(db - 1u - b) * B + boff] : 0u;
+      pc[b] = b < static_cast<int>(cb) ? s_dc[D + (cb - 1u - b) * B + boff] : 0u;
     }
-    pin_reads(pd);
-    pin_reads(pc);
     uint32_t dv = 0, cv = 0;
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
