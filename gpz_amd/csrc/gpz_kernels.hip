@@ -1431,6 +1431,8 @@ struct DecodeArgs {
   void* z;
 };
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // Point coordinate from its doubled code q2 = (q << 1) | 1
 // (src/cudaWarpCompress.cu:1003-1010): fma(q2, eb, min).
 template <typename T, typename Q>
@@ -1810,7 +1812,12 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
         const uint32_t i = lane + 64u * j;
-        d4[i] = sv[i + (i >> 4)];
+        // streaming stores: the output is written once and not read back
+        // by this kernel, and cached copies would only be written back
+        // while the next kernel runs
+        const uint4 v = sv[i + (i >> 4)];
+        const u32x4 nv = {v.x, v.y, v.z, v.w};
+        __builtin_nontemporal_store(nv, reinterpret_cast<u32x4*>(d4 + i));
       }
     };
     put(ox, a.x);
