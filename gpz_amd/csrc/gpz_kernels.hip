@@ -603,6 +603,17 @@ __device__ __forceinline__ uint32_t pad_byte(uint32_t p) {
   return a;
 }
 
+// Global-memory pointers.  The group bodies take their buffers from an
+// argument struct; in a batch that struct is read from a device array, and
+// a pointer read from memory has an unknown address space, so every access
+// through it would be a flat access (which also makes every LDS wait wait
+// for global loads).  The casts state what the buffers are.
+#define GPZ_GLOBAL __attribute__((address_space(1)))
+template <typename P>
+__device__ __forceinline__ GPZ_GLOBAL P* gbl(P* p) {
+  return (GPZ_GLOBAL P*)p;
+}
+
 // Quantised code of one coordinate: sat_u32 / sat_u64 of fma(x, 1/(2eb),
 // -min/(2eb)) (src/cudaWarpCompress.cu:179-184,
 // src/cudaWarpCompressDouble.cu:186-193).  v_cvt_u32_f32 truncates,
@@ -724,7 +735,7 @@ __device__ __forceinline__ void radix_pass(uint32_t (&key)[16], uint32_t (&val)[
 template <int kMaxBits>
 __device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32_t nb,
                                                  uint32_t B, uint32_t padded,
-                                                 uint8_t* dst, uint32_t lane) {
+                                                 GPZ_GLOBAL uint8_t* dst, uint32_t lane) {
   const bool w0 = 2u * lane < B, w1 = 2u * lane + 1u < B;
   if (kMaxBits <= 16 || nb <= 16) {
     uint32_t r[8];
@@ -735,7 +746,7 @@ __device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32
     for (int b = 0; b < 16; ++b) {
       if (b < static_cast<int>(nb)) {
         const uint32_t w = plane_word16(r, b);
-        uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
+        GPZ_GLOBAL uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
         if (w0) d[0] = static_cast<uint8_t>(w);
         if (w1) d[1] = static_cast<uint8_t>(w >> 8);
       }
@@ -749,7 +760,7 @@ __device__ __forceinline__ void store_msb_planes(const uint32_t (&v)[16], uint32
     for (int b = 0; b < kMaxBits; ++b) {
       if (b < static_cast<int>(nb)) {
         const uint32_t w = plane_word(t16, b);
-        uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
+        GPZ_GLOBAL uint8_t* d = dst + (nb - 1u - b) * B + 2u * lane;
         if (w0) d[0] = static_cast<uint8_t>(w);
         if (w1) d[1] = static_cast<uint8_t>(w >> 8);
       }
@@ -840,7 +851,8 @@ __device__ __forceinline__ void quantise(const T (&px)[16], const T (&py)[16],
 // Sums and products are taken in the same 32-bit wrap-around arithmetic as
 // quantise(), whose result they equal.
 template <typename T, int DIM, bool kFull, bool kFast>
-__device__ __forceinline__ void quantise_axes(const T* X, const T* Y, const T* Z,
+__device__ __forceinline__ void quantise_axes(const GPZ_GLOBAL T* X, const GPZ_GLOBAL T* Y,
+                                              const GPZ_GLOBAL T* Z,
                                               T inv, const T (&nmi)[3],
                                               uint32_t seg, uint32_t nbx32, uint32_t nbxy32,
                                               uint32_t jbase, uint32_t nvalid,
@@ -853,7 +865,7 @@ __device__ __forceinline__ void quantise_axes(const T* X, const T* Y, const T* Z
   const uint32_t mz = 2u * sh >= 32 ? 0u : m;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    const T* P = d == 0 ? X : d == 1 ? Y : Z;
+    const GPZ_GLOBAL T* P = d == 0 ? X : d == 1 ? Y : Z;
     uint32_t jb = jbase;
     asm volatile("" : "+v"(jb));  // a fresh read, not the registers of the first
     T v[16];
@@ -918,7 +930,7 @@ template <typename T, int DIM, bool kFull>
 __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
                                              uint32_t lane, uint8_t* lds) {
   using Q = typename Prec<T>::Q;
-  const uint32_t seg = a.info->seg_size;
+  const uint32_t seg = gbl(a.info)->seg_size;
   const T eb = static_cast<T>(a.eb);
   const uint64_t base = c * kGroup;
   const uint32_t nvalid = kFull ? kGroup : a.last_n;
@@ -931,9 +943,9 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   // keys are replaced below.
   T px[16], py[16], pz[16];
   {
-    const T* X = static_cast<const T*>(a.x) + base;
-    const T* Y = static_cast<const T*>(a.y) + base;
-    const T* Z = static_cast<const T*>(a.z) + base;
+    const GPZ_GLOBAL T* X = gbl(static_cast<const T*>(a.x)) + base;
+    const GPZ_GLOBAL T* Y = gbl(static_cast<const T*>(a.y)) + base;
+    const GPZ_GLOBAL T* Z = gbl(static_cast<const T*>(a.z)) + base;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       uint32_t j = jbase + 32u * k;
@@ -991,9 +1003,9 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   }
   uint32_t key[16], val[16], bor, ror;
   if constexpr (sizeof(T) == 8) {
-    const T* X = static_cast<const T*>(a.x) + base;
-    const T* Y = static_cast<const T*>(a.y) + base;
-    const T* Z = DIM == 3 ? static_cast<const T*>(a.z) + base : nullptr;
+    const GPZ_GLOBAL T* X = gbl(static_cast<const T*>(a.x)) + base;
+    const GPZ_GLOBAL T* Y = gbl(static_cast<const T*>(a.y)) + base;
+    const GPZ_GLOBAL T* Z = DIM == 3 ? gbl(static_cast<const T*>(a.z)) + base : nullptr;
     if (fast)
       quantise_axes<T, DIM, kFull, true>(X, Y, Z, inv, nmi, seg, nbx32, nbxy32, jbase,
                                          nvalid, key, val, &bor, &ror);
@@ -1095,7 +1107,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
   wave_sync();
 
   // ---- stage 4b: delta and count planes, MSB-first over run index
-  uint8_t* slot = a.slots + c * static_cast<uint64_t>(kSlot);
+  GPZ_GLOBAL uint8_t* slot = gbl(a.slots) + c * static_cast<uint64_t>(kSlot);
   uint32_t db, cb, D, C, R;
   if (small) {
     // One run per lane, lane l holding run l^7: a ballot of bit b is then
@@ -1133,7 +1145,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
       }
     }
     wave_sync();
-    uint32_t* sl32 = reinterpret_cast<uint32_t*>(slot);
+    GPZ_GLOBAL uint32_t* sl32 = reinterpret_cast<GPZ_GLOBAL uint32_t*>(slot);
     if (lane < nw) sl32[lane] = st32[lane];
     if (lane + 64 < nw) sl32[lane + 64] = st32[lane + 64];
   } else {
@@ -1175,7 +1187,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 
   // ---- stage 4c: relative-position planes, LSB-first over sorted rank
   {
-    uint8_t* pr = slot + D + C;
+    GPZ_GLOBAL uint8_t* pr = slot + D + C;
     if (rb <= 16) {
       uint32_t r[8];
 #pragma unroll
@@ -1184,7 +1196,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 #pragma unroll
       for (int b = 0; b < 16; ++b) {
         if (b < static_cast<int>(rb))
-          *reinterpret_cast<uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
+          *reinterpret_cast<GPZ_GLOBAL uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
               static_cast<uint16_t>(plane_word16(r, b));
       }
     } else {
@@ -1195,7 +1207,7 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
 #pragma unroll
       for (int b = 0; b < 32; ++b) {
         if (b < static_cast<int>(rb))
-          *reinterpret_cast<uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
+          *reinterpret_cast<GPZ_GLOBAL uint16_t*>(pr + (rb - 1u - b) * 128u + 2u * lane) =
               static_cast<uint16_t>(plane_word(t16, b));
       }
     }
@@ -1233,9 +1245,9 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
       }
     }
     if (lane < M / 4)
-      reinterpret_cast<uint32_t*>(a.meta + c * M)[lane] = word;
+      reinterpret_cast<GPZ_GLOBAL uint32_t*>(gbl(a.meta) + c * M)[lane] = word;
   }
-  if (lane == 0) a.loc[c] = D + C + R;
+  if (lane == 0) gbl(a.loc)[c] = D + C + R;
 }
 
 // One wave per group.  (A variant that prefetched the next group's
@@ -1390,11 +1402,11 @@ template <typename T>
 __device__ __forceinline__ void compact_group(const CompactArgs& a, uint64_t c,
                                               uint32_t lane) {
   constexpr uint32_t P = Prec<T>::kHeader, M = Prec<T>::kMeta;
-  uint8_t* pay = a.stream + P + (8ull + M) * a.G;
-  const uint32_t words = static_cast<uint32_t>(a.loc[c]) >> 2;
-  const uint32_t* src =
-      reinterpret_cast<const uint32_t*>(a.slots + c * static_cast<uint64_t>(kSlot));
-  uint32_t* dst = reinterpret_cast<uint32_t*>(pay + a.off[c]);
+  GPZ_GLOBAL uint8_t* pay = gbl(a.stream) + P + (8ull + M) * a.G;
+  const uint32_t words = static_cast<uint32_t>(gbl(a.loc)[c]) >> 2;
+  const GPZ_GLOBAL uint32_t* src =
+      reinterpret_cast<const GPZ_GLOBAL uint32_t*>(gbl(a.slots) + c * static_cast<uint64_t>(kSlot));
+  GPZ_GLOBAL uint32_t* dst = reinterpret_cast<GPZ_GLOBAL uint32_t*>(pay + gbl(a.off)[c]);
   for (uint32_t i0 = 0; i0 < words; i0 += 512) {
     uint32_t v[8];
 #pragma unroll
@@ -1479,7 +1491,7 @@ __device__ __forceinline__ void pin_reads(uint32_t (&v)[N]) {
 // the reads are in flight together (a use inside the branch makes every
 // plane wait for the one before).  With B even the two bytes are one
 // aligned 16-bit read; lanes past the plane (2l >= B) re-read byte 0.
-__device__ __forceinline__ void read_plane_pairs(const uint8_t* sec, uint32_t nb, uint32_t B,
+__device__ __forceinline__ void read_plane_pairs(const GPZ_GLOBAL uint8_t* sec, uint32_t nb, uint32_t B,
                                                  uint32_t lane, int b0, uint32_t (&h)[16]) {
   const uint32_t o0 = 2u * lane < B ? 2u * lane : 0u;
   const uint32_t o1 = 2u * lane + 1u < B ? 2u * lane + 1u : 0u;
@@ -1489,7 +1501,7 @@ __device__ __forceinline__ void read_plane_pairs(const uint8_t* sec, uint32_t nb
 #pragma unroll
     for (int j = 0; j < 16; ++j)
       if (b0 + j < static_cast<int>(nb))
-        h[j] = *reinterpret_cast<const uint16_t*>(sec + (nb - 1u - (b0 + j)) * B + o0);
+        h[j] = *reinterpret_cast<const GPZ_GLOBAL uint16_t*>(sec + (nb - 1u - (b0 + j)) * B + o0);
     pin_reads(h);
   } else {
     uint32_t hi[16];
@@ -1520,14 +1532,16 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
   uint32_t* s_map = reinterpret_cast<uint32_t*>(lds + kStartB);
 
   const uint64_t c = c32;
-  const uint8_t* meta = a.stream + P + 8 * a.G + M * c;
+  const GPZ_GLOBAL uint8_t* stream = gbl(a.stream);
+  const GPZ_GLOBAL uint8_t* meta = stream + P + 8 * a.G + M * c;
   T mm[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) mm[i] = reinterpret_cast<const T*>(meta)[i];
-  const Q outlier = *reinterpret_cast<const Q*>(meta + 6 * sizeof(T));
-  const uint16_t* u16 = reinterpret_cast<const uint16_t*>(meta + 6 * sizeof(T) + sizeof(Q));
+  for (int i = 0; i < 6; ++i) mm[i] = reinterpret_cast<const GPZ_GLOBAL T*>(meta)[i];
+  const Q outlier = *reinterpret_cast<const GPZ_GLOBAL Q*>(meta + 6 * sizeof(T));
+  const GPZ_GLOBAL uint16_t* u16 =
+      reinterpret_cast<const GPZ_GLOBAL uint16_t*>(meta + 6 * sizeof(T) + sizeof(Q));
   const uint32_t db = u16[0], cb = u16[1], rb = u16[2], U = u16[3];
-  const uint64_t off = reinterpret_cast<const uint64_t*>(a.stream + P)[c];
+  const uint64_t off = reinterpret_cast<const GPZ_GLOBAL uint64_t*>(stream + P)[c];
   uint32_t D, C, R;
   // FP64-2D streams carry a 4-byte-padded count section (encoder layout).
   section_sizes<T, DIM>(U, db, cb, rb, &D, &C, &R);
@@ -1537,7 +1551,7 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
   if (U == 0 || U > kGroup || db > 32 || cb > 16 || rb > 32 ||
       pay0 + D + C + R > a.stream_bytes)
     return;
-  const uint8_t* pay = a.stream + pay0;
+  const GPZ_GLOBAL uint8_t* pay = stream + pay0;
   const uint32_t B = (U + 7u) >> 3;
 
   // Groups of <= 64 runs with <= 16 delta bits hold one run per lane.
@@ -1549,15 +1563,15 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
   const bool small = U <= 64 && db <= 16;
   uint32_t dcw = 0;
   if (small)
-    dcw = *reinterpret_cast<const uint32_t*>(pay + (4u * lane < D + C ? 4u * lane : 0u));
+    dcw = *reinterpret_cast<const GPZ_GLOBAL uint32_t*>(pay + (4u * lane < D + C ? 4u * lane : 0u));
 
   // ---- relative positions of ranks 16l..16l+15 (LSB-first planes, one
   // 16-bit word per lane and plane)
   uint32_t rel[16];
   {
-    const uint8_t* pr = pay + D + C + 2u * lane;
+    const GPZ_GLOBAL uint8_t* pr = pay + D + C + 2u * lane;
     auto plane16 = [&](int b) -> uint32_t {
-      return *reinterpret_cast<const uint16_t*>(pr + (rb - 1u - b) * 128u);
+      return *reinterpret_cast<const GPZ_GLOBAL uint16_t*>(pr + (rb - 1u - b) * 128u);
     };
     if (rb <= 16) {
       uint32_t w[16];
@@ -1808,7 +1822,7 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
         sv[i + (i >> 4)] = w;
       }
       wave_sync();
-      uint4* d4 = reinterpret_cast<uint4*>(static_cast<T*>(dst) + base);
+      GPZ_GLOBAL uint4* d4 = reinterpret_cast<GPZ_GLOBAL uint4*>(gbl(static_cast<T*>(dst)) + base);
 #pragma unroll
       for (int j = 0; j < NV; ++j) {
         const uint32_t i = lane + 64u * j;
@@ -1817,16 +1831,16 @@ __device__ __forceinline__ void decode_group(const DecodeArgs& a, uint32_t c32,
         // while the next kernel runs
         const uint4 v = sv[i + (i >> 4)];
         const u32x4 nv = {v.x, v.y, v.z, v.w};
-        __builtin_nontemporal_store(nv, reinterpret_cast<u32x4*>(d4 + i));
+        __builtin_nontemporal_store(nv, reinterpret_cast<GPZ_GLOBAL u32x4*>(d4 + i));
       }
     };
     put(ox, a.x);
     put(oy, a.y);
     if (DIM == 3) put(oz, a.z);
   } else {
-    T* X = static_cast<T*>(a.x) + base + 16u * lane;
-    T* Y = static_cast<T*>(a.y) + base + 16u * lane;
-    T* Z = static_cast<T*>(a.z) + base + 16u * lane;
+    GPZ_GLOBAL T* X = gbl(static_cast<T*>(a.x)) + base + 16u * lane;
+    GPZ_GLOBAL T* Y = gbl(static_cast<T*>(a.y)) + base + 16u * lane;
+    GPZ_GLOBAL T* Z = gbl(static_cast<T*>(a.z)) + base + 16u * lane;
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       if (16u * lane + k < nvalid) {
@@ -1929,7 +1943,11 @@ __global__ __launch_bounds__(256) void batch_seg_decide_kernel(const BatchItem* 
 // The lane index, the LDS slice and the descriptor pointer are made opaque
 // in every iteration of a persistent loop: otherwise the compiler hoists
 // every lane-dependent value of the group body out of the loop and spills.
-#define GPZ_OPAQUE_ITER(ap, ln, lp) asm volatile("" : "+s"(ap), "+v"(ln), "+s"(lp))
+// The slice and the descriptor are made opaque as an LDS byte offset and a
+// snapshot index, not as pointers: an opaque pointer loses its address
+// space, and every LDS access of the group body, and every global access
+// through the pointers of the descriptor, becomes a flat access.
+#define GPZ_OPAQUE_ITER(item, ln, loff) asm volatile("" : "+s"(item), "+v"(ln), "+s"(loff))
 
 // Up to 8 waves per workgroup, each with its own kLdsBytes of dynamic LDS.
 // Three waves per SIMD: the loop state does not fit the 128 registers of
@@ -1941,14 +1959,14 @@ void batch_encode_kernel(const BatchItem* __restrict__ items,
   extern __shared__ __align__(16) uint8_t dyn_lds[];
   const uint32_t lane = threadIdx.x & 63u, w = wave_in_block();
   const uint32_t nw = blockDim.x >> 6;
-  uint8_t* lds = dyn_lds + w * kLdsBytes;
+  const uint32_t lds_off = w * kLdsBytes;
   for (uint32_t s = blockIdx.x * nw + w; s < nsegs; s += gridDim.x * nw) {
     const Segment sg = segs[s];
     for (uint32_t g = sg.g0; g < sg.g0 + sg.ng; ++g) {
-      const EncodeArgs* ap = &items[sg.item].e;
-      uint32_t ln = lane;
-      uint8_t* lp = lds;
-      GPZ_OPAQUE_ITER(ap, ln, lp);
+      uint32_t item = sg.item, ln = lane, loff = lds_off;
+      GPZ_OPAQUE_ITER(item, ln, loff);
+      const EncodeArgs* ap = &items[item].e;
+      uint8_t* lp = dyn_lds + loff;
       if (ap->info->status != GPZ_OK) break;
       encode_one<T, DIM>(*ap, g, ln, lp);
     }
@@ -2010,14 +2028,14 @@ void batch_decode_kernel(const DecodeArgs* __restrict__ items,
   extern __shared__ __align__(16) uint8_t dyn_lds[];
   const uint32_t lane = threadIdx.x & 63u, w = wave_in_block();
   const uint32_t nw = blockDim.x >> 6;
-  uint8_t* lds = dyn_lds + w * dec_lds_bytes<T>();
+  const uint32_t lds_off = w * dec_lds_bytes<T>();
   for (uint32_t s = blockIdx.x * nw + w; s < nsegs; s += gridDim.x * nw) {
     const Segment sg = segs[s];
     for (uint32_t g = sg.g0; g < sg.g0 + sg.ng; ++g) {
-      const DecodeArgs* ap = &items[sg.item];
-      uint32_t ln = lane;
-      uint8_t* lp = lds;
-      GPZ_OPAQUE_ITER(ap, ln, lp);
+      uint32_t item = sg.item, ln = lane, loff = lds_off;
+      GPZ_OPAQUE_ITER(item, ln, loff);
+      const DecodeArgs* ap = &items[item];
+      uint8_t* lp = dyn_lds + loff;
       decode_group<T, DIM>(*ap, g, ln, lp);
     }
   }
