@@ -1950,10 +1950,11 @@ __global__ __launch_bounds__(256) void batch_seg_decide_kernel(const BatchItem* 
 #define GPZ_OPAQUE_ITER(item, ln, loff) asm volatile("" : "+s"(item), "+v"(ln), "+s"(loff))
 
 // Up to 8 waves per workgroup, each with its own kLdsBytes of dynamic LDS.
-// Three waves per SIMD: the loop state does not fit the 128 registers of
-// the single-group kernel without spilling.
+// FP32 at four waves per SIMD, as the single-group kernel: the loop state
+// spills some registers at 128, and the mix still ran 4 % faster than at
+// three waves without spills.
 template <typename T, int DIM>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 void batch_encode_kernel(const BatchItem* __restrict__ items,
                          const Segment* __restrict__ segs, uint32_t nsegs) {
   extern __shared__ __align__(16) uint8_t dyn_lds[];
