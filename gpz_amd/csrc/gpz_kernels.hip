@@ -1048,6 +1048,36 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
       key[k] >>= 10;
     }
     // invalid ranks keep all-ones block ids (never compared below)
+  } else if (((sortbits - 19u) & ~3u) + rb <= 32u) {
+    // 23..32 key bits: pair passes (keys and values move) over the lowest
+    // `lo` bits only, until <= 22 bits are left; then the packed passes
+    // with the rank after the pair passes in the low 10 bits.  The low
+    // block-id bits ride above the rb value bits meanwhile (rb + lo <= 32),
+    // and one gather by that rank brings both back.
+    const uint32_t lo = (sortbits - 19u) & ~3u;   // 4, 8 or 12
+    for (uint32_t shift = 0; shift < lo; shift += 4)
+      radix_pass<true>(key, val, shift, lane, sl, lds);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      val[k] |= (key[k] & ((1u << lo) - 1u)) << rb;
+      key[k] = ((key[k] >> lo) << 10) | (16u * lane + k);
+    }
+    for (uint32_t shift = 10; shift < 10 + sortbits - lo; shift += 4)
+      radix_pass<false>(key, val, shift, lane, sl, lds);
+    wave_sync();
+    uint32_t* s_val = reinterpret_cast<uint32_t*>(lds + kGatherB);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s_val[17 * lane + k] = val[k];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t v =
+          *reinterpret_cast<const uint32_t*>(lds + kGatherB + pad_byte(key[k] & 1023u));
+      key[k] = ((key[k] >> 10) << lo) | (v >> rb);
+      val[k] = v & ((1u << rb) - 1u);
+    }
+    // invalid ranks: all-ones values in the rb bits the planes use; their
+    // block ids are never compared below
   } else {
     for (uint32_t shift = 0; shift < sortbits; shift += 4)
       radix_pass<true>(key, val, shift, lane, sl, lds);
