@@ -219,3 +219,25 @@ def test_all_distinct_blocks(oracle, bits, dim):
     from gpz_amd import api
     h = api.parse_header(s_o)
     assert h.groups == 5
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("dim,e,seg,sortbits", [
+    (3, 1.0, 2, 24),      # pair pass over 4 low bits, then packed passes
+    (3, 0.4, 2, 28),      # 8 low bits
+    (2, 0.0076, 4, 28),   # 2D, 4 rel bits
+    (3, 0.02, 4, 31),     # 12 low bits (31-bit masked block ids)
+])
+def test_wide_sort_keys(oracle, bits, dim, e, seg, sortbits):
+    """Groups whose block ids need 23-32 sort bits (the encoder's pair-then-
+    packed sort), uniform in [0, 1000)^dim, with a partial last group."""
+    rng = np.random.default_rng(sortbits * 10 + dim)
+    n = 5 * 1024 + 77
+    dt = np.float64 if bits == 64 else np.float32
+    pts = [rng.uniform(0, 1000, n).astype(dt) for _ in range(dim)]
+    x, y = pts[0], pts[1]
+    z = pts[2] if dim == 3 else None
+    # the workload reaches the intended sort width in its first group
+    nb = math.ceil((float(x[:1024].max() - x[:1024].min())) / (2 * seg * e)) + 1
+    assert (nb ** dim).bit_length() >= sortbits - 1
+    _parity(oracle, x, y, z, "abs", e, seg_in=seg)
