@@ -1295,8 +1295,11 @@ __device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t c,
     encode_group<T, DIM, false>(a, c, lane, lds);
 }
 
+// FP32 3D at 4 waves per SIMD (its wide-key paths spill; the common
+// <= 22-bit path does not); FP32 2D, whose uniform groups mostly take the
+// wide-key path, and FP64 at 3 without spills.
 template <typename T, int DIM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && DIM == 3 ? 4 : 3)))
 void encode_kernel(EncodeArgs a) {
   __shared__ __align__(16) uint8_t lds[kLdsBytes];
   if (a.info->status != GPZ_OK) return;
