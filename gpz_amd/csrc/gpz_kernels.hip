@@ -262,6 +262,23 @@ __device__ __forceinline__ float wave_ext(float v) {
   return __uint_as_float(lane63(__float_as_uint(v)));
 }
 
+// A double that is equal in every lane, read from the first active lane
+// (so that the compiler can keep it in scalar registers).
+template <typename V>
+__device__ __forceinline__ V uniform_f64(V v) {
+  if constexpr (sizeof(V) == 8) {
+    const uint64_t u = static_cast<uint64_t>(__double_as_longlong(static_cast<double>(v)));
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(u))));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(u >> 32))));
+    return static_cast<V>(__longlong_as_double(static_cast<long long>(
+        (static_cast<uint64_t>(hi) << 32) | lo)));
+  } else {
+    return v;
+  }
+}
+
 // Bit-matrix transposes.  One swap stage exchanges the m-masked bits of
 // a >> j with those of b (two shifts and two bitfield inserts).
 __device__ __forceinline__ void swap_bits(uint32_t& a, uint32_t& b, int j,
@@ -972,8 +989,9 @@ __device__ __forceinline__ void encode_group(const EncodeArgs& a, uint64_t c,
       mn[d] = wave_ext<false>(mn[d]);
       mx[d] = wave_ext<true>(mx[d]);
     } else {
-      mn[d] = wave_min(mn[d]);
-      mx[d] = wave_max(mx[d]);
+      // wave-uniform: kept in scalar registers through the sort
+      mn[d] = uniform_f64(wave_min(mn[d]));
+      mx[d] = uniform_f64(wave_max(mx[d]));
     }
   }
   if (DIM == 2) { mn[2] = T(0); mx[2] = T(0); }
